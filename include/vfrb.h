@@ -1,0 +1,389 @@
+/*
+ * vfrb.h - C ABI of the MI355X (gfx950) VLITE FRB search chain.
+ *
+ * One shared library, libvfrb.so, exports every function declared here.
+ * Plain C types only: device buffers are `void*`/typed pointers obtained from
+ * vfrb_malloc (or any HIP allocation), streams are opaque `void*` (a
+ * hipStream_t; NULL = the legacy default stream).  Every call returns a
+ * vfrb_status (0 = success) and never exits the process; the message of the
+ * last failure on the calling thread is returned by vfrb_last_error().
+ *
+ * Each entry point names the reference code it replaces (paths relative to
+ * the paulray/vlite-frb tree):
+ *   PIPELINE   = vdifFILE2dedisp_PIPELINE.cu
+ *   TIMESERIES = vdifDADA2timeseries.cu
+ *   STATS      = vdifDADA2stats.cu
+ *
+ * Threading: plans are not thread safe; use one plan per host thread.  All
+ * device work is enqueued asynchronously on the stream argument.
+ */
+#ifndef VFRB_H
+#define VFRB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+typedef int32_t vfrb_status;
+#define VFRB_OK              0
+#define VFRB_ERR_INVALID     1  /* bad argument or inconsistent shape        */
+#define VFRB_ERR_UNSUPPORTED 2  /* shape the kernels do not implement        */
+#define VFRB_ERR_NOMEM       3  /* device or host allocation failed          */
+#define VFRB_ERR_HIP         4  /* HIP runtime / launch error                */
+#define VFRB_ERR_NODEVICE    5  /* no usable GPU                             */
+#define VFRB_ERR_IO          6  /* file I/O error                            */
+#define VFRB_ERR_FORMAT      7  /* VDIF stream violates the expected layout  */
+#define VFRB_ERR_EOF         8  /* end of input reached                      */
+#define VFRB_ERR_CAPACITY    9  /* request exceeds a fixed capacity          */
+
+/* Replaces HandleError()'s message text (PIPELINE:124-132). */
+const char* vfrb_strerror(vfrb_status status);
+/* Message of the last failure on this thread ("" if none). */
+const char* vfrb_last_error(void);
+const char* vfrb_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Derived parameters (host)                                                */
+/* ------------------------------------------------------------------------ */
+/* VLITE constants, PIPELINE:33-47. */
+#define VFRB_FRAMELEN     5032
+#define VFRB_HDRLEN       32
+#define VFRB_DATALEN      5000
+#define VFRB_FRAMESPERSEC 25600
+#define VFRB_SAMPLERATE   128e6
+#define VFRB_BW_MHZ       64.0
+#define VFRB_LOWFREQ_MHZ  320.0
+#define VFRB_MUOSLOW_MHZ  360.0
+
+typedef struct vfrb_params {
+    int32_t twonchan;  /* FFT length (2*NCHAN)                             */
+    int32_t nchan;     /* NCHAN                                             */
+    int32_t nfreq;     /* NCHAN+1 (R2C bins)                                */
+    int32_t nave;      /* spectra averaged per output sample                */
+    int32_t nbundle;   /* output samples per host bundle                    */
+    int32_t nloop;     /* bytes per pol per bundle = NBUNDLE*NAVE*TWONCHAN  */
+    int32_t nchanout;  /* channels kept                                     */
+    int32_t nskip;     /* first FFT bin kept (channel 0 = highest freq)     */
+    int32_t maxnt;     /* rows per dedispersion window                      */
+    int32_t nhalfcyc;  /* 128-row bandpass blocks per window                */
+    double chanbw;     /* MHz                                               */
+    double foff;       /* -chanbw (SIGPROC)                                 */
+    double fch1;       /* centre of channel 0, MHz                          */
+    double tchan;      /* output sample time, s                             */
+    double tbundle;    /* bundle duration, s                                */
+    double maxtime;    /* window length, s                                  */
+} vfrb_params;
+
+/*
+ * Parameter derivation of PIPELINE:289-309 (same formulae in
+ * TIMESERIES:276-305, STATS:209-232).  fullband=0 keeps the reference's
+ * MUOS cut (MUOSLOWFREQ = 360 MHz); fullband=1 keeps all NCHAN channels
+ * (NCHANOUT = NCHAN, NSKIP = 1), the shape of the 1024- and 4096-channel
+ * workloads.  nchanout_override > 0 forces NCHANOUT (TIMESERIES:282 uses
+ * 3840) with NSKIP = NCHAN - NCHANOUT + 1.
+ */
+vfrb_status vfrb_params_derive(int32_t twonchan, int32_t nave, int32_t nbundle,
+                               double maxtime, int32_t fullband,
+                               int32_t nchanout_override, vfrb_params* out);
+
+/*
+ * Dispersion delay table, FP64 then truncated to int (bit-exact with the
+ * reference formula, TIMESERIES:449-464; the same text is commented at
+ * PIPELINE:446-461):
+ *   off[i][0] = 0,
+ *   off[i][j] = (int)(4.148808e3*DM[i]*(pow(fch1-j*chanbw,-2)-pow(fch1,-2))/tchan)
+ *   max_delay = same formula at j = nchanout for the LAST DM of the list.
+ * offsets may be NULL (only max_delay wanted); max_delay may be NULL.
+ * min_margin (optional) receives the smallest distance of dt/tchan to an
+ * integer over the table, i.e. how far any delay is from flipping.
+ */
+vfrb_status vfrb_delay_table(const float* dms, int32_t ndm, int32_t nchanout,
+                             double fch1, double chanbw, double tchan,
+                             int32_t* offsets, int32_t* max_delay,
+                             double* min_margin);
+
+/* ------------------------------------------------------------------------ */
+/* Device runtime plumbing (thin wrappers over HIP)                         */
+/* ------------------------------------------------------------------------ */
+vfrb_status vfrb_device_count(int32_t* count);
+vfrb_status vfrb_set_device(int32_t device);
+vfrb_status vfrb_device_name(char* buf, int32_t cap);
+vfrb_status vfrb_mem_info(size_t* free_bytes, size_t* total_bytes);
+vfrb_status vfrb_malloc(void** ptr, size_t bytes);
+vfrb_status vfrb_free(void* ptr);
+vfrb_status vfrb_host_alloc(void** ptr, size_t bytes);  /* pinned */
+vfrb_status vfrb_host_free(void* ptr);
+vfrb_status vfrb_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+vfrb_status vfrb_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+vfrb_status vfrb_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+vfrb_status vfrb_memset(void* dst, int32_t value, size_t bytes, void* stream);
+vfrb_status vfrb_stream_create(void** stream);
+vfrb_status vfrb_stream_destroy(void* stream);
+vfrb_status vfrb_stream_synchronize(void* stream);
+vfrb_status vfrb_device_synchronize(void);
+vfrb_status vfrb_event_create(void** event);
+vfrb_status vfrb_event_destroy(void* event);
+vfrb_status vfrb_event_record(void* event, void* stream);
+vfrb_status vfrb_event_synchronize(void* event);
+vfrb_status vfrb_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ------------------------------------------------------------------------ */
+/* Front end: fused unpack + R2C FFT + detection + pol sum + time average   */
+/* ------------------------------------------------------------------------ */
+typedef struct vfrb_fb_plan vfrb_fb_plan;
+
+/*
+ * Replaces cufftPlan1d(&plan, TWONCHAN, CUFFT_R2C, NAVE*NBUNDLE)
+ * (PIPELINE:409).  Power-of-two twonchan in [256, 16384] runs the
+ * specialised radix-8 kernel; other even lengths whose factors are in
+ * {2,3,4,5,8} (e.g. the reference default 12500) run the mixed-radix kernel.
+ */
+vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan,
+                                int32_t nave, int32_t nskip, int32_t nchanout);
+vfrb_status vfrb_fb_plan_destroy(vfrb_fb_plan* plan);
+
+/*
+ * Replaces, per bundle, convertarray x2 + cufftExecR2C x2 + sumpower +
+ * avepower + the NBUNDLE channel-select copies (PIPELINE:582-632,
+ * 1292-1322).  pol0/pol1 hold nrows*nave*twonchan 8-bit offset-binary
+ * samples each (device).  Output row r goes to
+ *   fb[((row0 + r) mod ring_rows) * nchanout + c],  c < nchanout,
+ * with fb[.][c] = (1/nave) * sum_frames (|X0[nskip+c]|^2+|X1[nskip+c]|^2)/nfreq.
+ * ring_rows = 0 means a linear buffer (no wrap).
+ */
+vfrb_status vfrb_fb_execute(vfrb_fb_plan* plan, const uint8_t* pol0,
+                            const uint8_t* pol1, int64_t nrows, float* fb,
+                            int64_t ring_rows, int64_t row0, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Bandpass (time-major, in place)                                          */
+/* ------------------------------------------------------------------------ */
+/*
+ * subtractbpCAL (PIPELINE:1427-1474; flat variant TIMESERIES:1365-1407):
+ * for every channel and every block of 128 rows (last block may be short):
+ * mean, "sig" = variance (sic), clipped mean over (mean-3sig, mean+3sig),
+ * subtract the clipped mean.  fb is f32 [nrows][nch], device, in place.
+ */
+vfrb_status vfrb_bandpass_inplace(float* fb, int32_t nrows, int32_t nch, void* stream);
+
+/*
+ * Per-channel statistics of STATS calcstats (STATS:910-946): per channel and
+ * block of ncolstatblk rows, mean, variance (named sig), and the outlier
+ * fraction computed with integer division (count/npts).  Outputs are f32
+ * [nstatblks][nch].
+ */
+vfrb_status vfrb_chan_stats(const float* fb, int32_t nrows, int32_t nch,
+                            int32_t ncolstatblk, int32_t nstatblks,
+                            float* ave, float* sig, float* outlier, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Back end: search plan (replaces the dedisp plan)                         */
+/* ------------------------------------------------------------------------ */
+#define VFRB_STORE_F32 0  /* reference: f32 residuals                        */
+#define VFRB_STORE_F16 1  /* f16 residuals, window-global power-of-2 scale   */
+#define VFRB_STORE_U8  2  /* 8-bit, f32 per-channel scale and offset         */
+
+#define VFRB_ACC_F64   0  /* statistics accumulate in FP64 (reference)       */
+#define VFRB_ACC_F32C  1  /* compensated (Neumaier) FP32                     */
+
+typedef struct vfrb_search_desc {
+    int32_t nchans;       /* channels (NCHANOUT)                             */
+    double  tsamp;        /* s per row (tchan)                               */
+    double  fch1;         /* MHz, centre of channel 0 (highest frequency)    */
+    double  chanbw;       /* MHz, positive; channel j is fch1 - j*chanbw     */
+    int32_t ndm;
+    const float* dms;     /* host, ndm values                                */
+    int32_t max_rows;     /* largest window (rows) ever passed               */
+    int32_t storage;      /* VFRB_STORE_*                                    */
+    int32_t tile_config;  /* -1 = autotuned/default, else an index returned
+                             by vfrb_search_tile_count/vfrb_search_tile_name */
+} vfrb_search_desc;
+
+typedef struct vfrb_search_plan vfrb_search_plan;
+
+/*
+ * Replaces dedisp_create_plan + dedisp_set_dm_list + dedisp_get_dm_count +
+ * dedisp_get_max_delay (PIPELINE:416-441) and the in-tree OFFSET table
+ * (TIMESERIES:449-464).  The delay table is computed on the host in FP64
+ * (vfrb_delay_table) and kept on the device.
+ */
+vfrb_status vfrb_search_plan_create(vfrb_search_plan** plan, const vfrb_search_desc* desc);
+vfrb_status vfrb_search_plan_destroy(vfrb_search_plan* plan);
+int32_t     vfrb_search_max_delay(const vfrb_search_plan* plan);
+int32_t     vfrb_search_dm_count(const vfrb_search_plan* plan);
+/* Copies the [ndm][nchans] int32 delay table to host memory. */
+vfrb_status vfrb_search_get_offsets(const vfrb_search_plan* plan, int32_t* host_offsets);
+
+/*
+ * Loads one window into the plan: rows row0 .. row0+nrows-1 of a time-major
+ * f32 filterbank ring (fb[(row mod ring_rows)*nchans + c]; ring_rows = 0 =
+ * linear), applies subtractbpCAL when apply_bandpass != 0 (blocks of 128
+ * rows aligned to row0, PIPELINE:1427-1474), and stores the residuals
+ * channel-major in the plan's storage type.  This fuses PIPELINE:653-658
+ * (tail save + bandpass) with the layout change the dedispersion wants.
+ */
+vfrb_status vfrb_search_load(vfrb_search_plan* plan, const float* fb,
+                             int64_t ring_rows, int64_t row0, int32_t nrows,
+                             int32_t apply_bandpass, void* stream);
+
+/*
+ * Replaces dedisp_execute(plan, nrows, fb, 32, out, 32, DEVICE_POINTERS)
+ * (PIPELINE:671) with the arithmetic of the in-tree kernel `dedisperse`
+ * (PIPELINE:1517-1537):
+ *   out[dm][t] = (1/nchans) * sum_c FB[t + off[dm][c]][c],
+ *   t < nrows - max_delay,
+ * on the window last given to vfrb_search_load.  out is f32 [ndm][ld_out].
+ */
+vfrb_status vfrb_search_dedisperse(vfrb_search_plan* plan, int32_t nrows,
+                                   float* out, int64_t ld_out, void* stream);
+
+/*
+ * Convenience: dedisp_execute on a caller-provided, already bandpassed,
+ * time-major f32 filterbank (PIPELINE:671): load without bandpass, then
+ * dedisperse.
+ */
+vfrb_status vfrb_search_execute(vfrb_search_plan* plan, int32_t nrows,
+                                const float* fb_time_major, float* out,
+                                int64_t ld_out, void* stream);
+
+/*
+ * dedisperseSTATS (TIMESERIES:1450-1481): mean over channels and the
+ * channel-scatter term sqrt((S2 - S1*S1)/(N*(N-1))) with
+ * S1 = sum(x-mean), S2 = sum((x-mean)^2) (the reference omits /N on S1*S1).
+ * Works on f32 storage plans only.
+ */
+vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* plan, int32_t nrows,
+                                         float* out, float* sig, int64_t ld_out,
+                                         void* stream);
+
+/* Tile configurations of the dedispersion kernel (for the autotuner). */
+int32_t     vfrb_search_tile_count(void);
+const char* vfrb_search_tile_name(int32_t index);
+vfrb_status vfrb_search_set_tile(vfrb_search_plan* plan, int32_t index);
+int32_t     vfrb_search_get_tile(const vfrb_search_plan* plan);
+
+/* ------------------------------------------------------------------------ */
+/* Per-DM statistics and candidates                                         */
+/* ------------------------------------------------------------------------ */
+/*
+ * calcstats (PIPELINE:1337-1370): per DM, mean, two-pass corrected variance
+ * (sum2 - sum*sum/N)/(N-1), ave=(float)mean, sigma=(float)sqrt(var),
+ * SNR_t = (x_t - ave)/sigma in f32, peak = max(0, max SNR), frb = any SNR > 3.
+ * acc = VFRB_ACC_F64 (reference) or VFRB_ACC_F32C (compensated FP32).
+ * out is f32 [ndm][ld]; results are f32/int32 [ndm].  Any output pointer
+ * except ave/sigma may be NULL.
+ */
+vfrb_status vfrb_dm_stats(const float* out, int64_t ld, int32_t ndm, int32_t nt,
+                          int32_t acc, int32_t* frb, float* ave, float* sigma,
+                          float* peak, int32_t* peak_t, void* stream);
+
+/*
+ * Candidate list (no reference counterpart; the reference only flags
+ * SNR > 3 per DM): every (dm, t) with (x - ave[dm])/sigma[dm] > threshold
+ * (strict, as PIPELINE:1363), appended through an atomic counter.  *count
+ * (device int32, zeroed by this call on the stream) receives the total number
+ * found, which may exceed max_cand (then only max_cand entries are stored,
+ * in no particular order).
+ */
+vfrb_status vfrb_candidates(const float* out, int64_t ld, int32_t ndm, int32_t nt,
+                            const float* ave, const float* sigma, float threshold,
+                            int32_t* cand_dm, int32_t* cand_t, float* cand_snr,
+                            int32_t max_cand, int32_t* count, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* VDIF (host)                                                              */
+/* ------------------------------------------------------------------------ */
+/* Decoded fields of the 32-byte VDIF header (vdifio.h:53-72). */
+typedef struct vfrb_vdif_header {
+    uint32_t seconds, legacymode, invalid;
+    uint32_t frame, epoch, unassigned;
+    uint32_t framelength8, nchan, version;
+    uint32_t stationid, threadid, nbits, iscomplex;
+    uint32_t eversion, extended1, extended2, extended3, extended4;
+} vfrb_vdif_header;
+
+vfrb_status vfrb_vdif_decode(const uint8_t* raw32, vfrb_vdif_header* out);
+vfrb_status vfrb_vdif_encode(const vfrb_vdif_header* in, uint8_t* raw32);
+/* getVDIFDMJD (external libvfdifio, called at PIPELINE:525): MJD (days, with
+ * fraction) of the frame: epoch start + seconds + frame/framespersec. */
+double      vfrb_vdif_dmjd(const uint8_t* raw32, int32_t framespersec);
+
+/*
+ * Frame sequencer (PIPELINE:829-1171): alternating pol-0/pol-1 frames, thread
+ * alignment, pol frame-number agreement, start on the 10 Hz cal boundary
+ * (frame % 2560 == 0), gap detection by frame number (mod 25600) with
+ * zero-filled payloads, and partial-frame carry across bundles.
+ * The log callback, when set, receives the reference's log lines.
+ */
+typedef struct vfrb_seq vfrb_seq;
+typedef void (*vfrb_log_fn)(const char* line, void* user);
+
+vfrb_status vfrb_seq_open_file(vfrb_seq** seq, const char* path);
+vfrb_status vfrb_seq_open_memory(vfrb_seq** seq, const uint8_t* data, size_t bytes);
+vfrb_status vfrb_seq_set_log(vfrb_seq* seq, vfrb_log_fn fn, void* user);
+/* initframe01 + initskip01 (PIPELINE:1044-1131).  Returns VFRB_ERR_EOF at
+ * end of input and VFRB_ERR_FORMAT on an alignment failure. */
+vfrb_status vfrb_seq_init(vfrb_seq* seq);
+/* Header of the current pol-0 frame (after init: the first frame used). */
+vfrb_status vfrb_seq_header(const vfrb_seq* seq, uint8_t* raw32);
+/* filltimebufferHOST (PIPELINE:1135-1171): fills nloop bytes per pol.
+ * Returns VFRB_ERR_EOF when the bundle cannot be completed (the reference
+ * drops that bundle). */
+vfrb_status vfrb_seq_fill(vfrb_seq* seq, int32_t nloop, uint8_t* out0, uint8_t* out1);
+/* Frames read and frames zero-filled so far, per pol. */
+vfrb_status vfrb_seq_counters(const vfrb_seq* seq, int64_t* frames_read, int64_t* frames_filled);
+vfrb_status vfrb_seq_close(vfrb_seq* seq);
+
+/* ------------------------------------------------------------------------ */
+/* Output formats (host)                                                    */
+/* ------------------------------------------------------------------------ */
+/* SIGPROC header of PIPELINE:529-540 (send_* of PIPELINE:1177-1207) into buf;
+ * *len receives the byte count (needs <= 512 bytes). */
+vfrb_status vfrb_sigproc_header(uint8_t* buf, int32_t cap, int32_t* len,
+                                double fch1, double foff, int32_t nchans,
+                                int32_t nbits, double tstart, double tsamp);
+/* PRESTO .inf text of writeprestoheader (TIMESERIES:1483-1515). */
+vfrb_status vfrb_presto_inf(char* buf, int32_t cap, const char* basename,
+                            double mjdstart, int32_t ntimes, double tsamp,
+                            float dm, double fchanlo, double bw, int32_t nchan,
+                            double chanbw);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic VLITE voltages (host; counter-based RNG, any thread count)     */
+/* ------------------------------------------------------------------------ */
+#define VFRB_SYNTH_MAX_PULSES 64
+#define VFRB_SYNTH_MAX_TONES  64
+
+typedef struct vfrb_synth_spec {
+    uint64_t seed;
+    int32_t  nfft;        /* synthesis block length (= twonchan)             */
+    double   mean, sigma; /* 128, 8                                           */
+    double   f_top_mhz;   /* frequency of FFT bin 0 (384: inverted band)     */
+    double   bw_mhz;      /* 64                                               */
+    int32_t  npulse;
+    double   pulse_dm[VFRB_SYNTH_MAX_PULSES];
+    double   pulse_t0[VFRB_SYNTH_MAX_PULSES];    /* s, arrival at f_top      */
+    double   pulse_fwhm[VFRB_SYNTH_MAX_PULSES];  /* s                         */
+    double   pulse_amp[VFRB_SYNTH_MAX_PULSES];   /* fractional power excess   */
+    int32_t  ntone;
+    double   tone_bin[VFRB_SYNTH_MAX_TONES];     /* FFT bin (fractional ok)   */
+    double   tone_amp[VFRB_SYNTH_MAX_TONES];     /* counts (peak)             */
+    int32_t  nthreads;    /* 0 = all available                               */
+} vfrb_synth_spec;
+
+/* Samples [block0*nfft, (block0+nblocks)*nfft) of one polarisation. */
+vfrb_status vfrb_synth_payload(const vfrb_synth_spec* spec, int32_t pol,
+                               int64_t block0, int64_t nblocks, uint8_t* out);
+/* Pulse amplitude giving a dedispersed peak SNR (SNR ~ A*sqrt(2*nave*nchan)). */
+double      vfrb_synth_amp_for_snr(double snr, int32_t nave, int32_t nchan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VFRB_H */

@@ -1,0 +1,260 @@
+"""GPU parity: every HIP kernel against the CPU oracle on the same seeded
+inputs, through the C ABI (libvfrb.so)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _noise_bytes(rng, n):
+    return np.clip(np.rint(rng.normal(128, 8, n)), 0, 255).astype(np.uint8)
+
+
+def _rel_err(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b)) / np.mean(np.abs(b)))
+
+
+# ---------------------------------------------------------- front end ----
+@pytest.mark.parametrize("n,nave,nrows,fullband", [
+    (2048, 64, 6, True),     # 1024-channel shape
+    (8192, 16, 5, True),     # 4096-channel shape
+    (4096, 8, 7, False),     # MUOS cut
+    (1024, 5, 9, True),      # nave not a multiple of the FFTs in flight
+    (256, 3, 4, True),
+    (16384, 2, 3, True),
+    (12500, 2, 3, False),    # reference default length (mixed radix 8*5*5*5*5*... )
+    (240, 7, 5, True),       # radix 8*2*3*5
+])
+def test_filterbank_vs_oracle(gpu, orc, n, nave, nrows, fullband):
+    vfrb = gpu
+    p = vfrb.params_derive(n, nave, 1, 1.0, fullband=fullband)
+    print(f"[case] filterbank n={n} nave={nave} rows={nrows} nskip={p.nskip} nch={p.nchanout}", flush=True)
+    rng = np.random.default_rng(n + nave)
+    tot = nrows * nave * n
+    p0, p1 = _noise_bytes(rng, tot), _noise_bytes(rng, tot)
+    d0, d1 = vfrb.DeviceArray.from_host(p0), vfrb.DeviceArray.from_host(p1)
+    fb = vfrb.DeviceArray((nrows, p.nchanout), np.float32)
+    plan = vfrb.FilterbankPlan(n, nave, p.nskip, p.nchanout)
+    plan.execute(d0, d1, nrows, fb)
+    got = fb.download()
+    ref = orc.filterbank_f64(p0, p1, nrows, n, nave, p.nskip, p.nchanout)
+    # FP32 transform vs FP64: relative to the mean power of the row
+    assert _rel_err(got, ref) < 2e-4, _rel_err(got, ref)
+
+
+def test_filterbank_ring_wrap(gpu, orc):
+    vfrb = gpu
+    n, nave, nrows, ring = 512, 4, 6, 8
+    rng = np.random.default_rng(9)
+    p0, p1 = _noise_bytes(rng, nrows * nave * n), _noise_bytes(rng, nrows * nave * n)
+    d0, d1 = vfrb.DeviceArray.from_host(p0), vfrb.DeviceArray.from_host(p1)
+    fb = vfrb.DeviceArray((ring, n // 2), np.float32)
+    fb.zero()
+    plan = vfrb.FilterbankPlan(n, nave, 1, n // 2)
+    plan.execute(d0, d1, nrows, fb, ring_rows=ring, row0=5)
+    got = fb.download()
+    ref = orc.filterbank_f64(p0, p1, nrows, n, nave, 1, n // 2)
+    order = [(5 + r) % ring for r in range(nrows)]
+    assert _rel_err(got[order], ref) < 2e-4
+    untouched = sorted(set(range(ring)) - set(order))
+    assert np.all(got[untouched] == 0)
+
+
+def test_filterbank_tone_closed_form(gpu):
+    """Period-4 tone on pol 0: all power in bin N/4, |X| = a*N/2."""
+    vfrb = gpu
+    n, nave, nrows, a = 8192, 16, 2, 40
+    pat = np.tile(np.array([128 + a, 128, 128 - a, 128], np.uint8), nrows * nave * n // 4)
+    flat = np.full(nrows * nave * n, 128, np.uint8)
+    d0, d1 = vfrb.DeviceArray.from_host(pat), vfrb.DeviceArray.from_host(flat)
+    fb = vfrb.DeviceArray((nrows, n // 2), np.float32)
+    vfrb.FilterbankPlan(n, nave, 1, n // 2).execute(d0, d1, nrows, fb)
+    got = fb.download()
+    expect = (a * n / 2.0) ** 2 / (n // 2 + 1)
+    assert np.allclose(got[:, n // 4 - 1], expect, rtol=1e-5)
+    others = np.delete(got, n // 4 - 1, axis=1)
+    assert np.max(np.abs(others)) < 1e-6 * expect
+
+
+# ----------------------------------------------------------- bandpass ----
+def test_bandpass_inplace_vs_oracle(gpu, orc):
+    vfrb = gpu
+    rng = np.random.default_rng(1)
+    rows, nch = 976, 1024
+    fb = rng.normal(256, 22, (rows, nch)).astype(np.float32)
+    fb[:, 77] += 5000.0                # RFI channel
+    fb[300:310, 5] += 800.0            # burst
+    fb[:, 9] = 3.0                     # constant channel: left untouched
+    d = vfrb.DeviceArray.from_host(fb)
+    vfrb.bandpass_inplace(d, rows, nch)
+    got = d.download()
+    ref, margin = orc.bandpass_f32(fb, with_margin=True)
+    ok = np.repeat(margin > 1e-4, 128, axis=0)[:rows]    # blocks with no value at a clip edge
+    assert ok.mean() > 0.99
+    assert np.max(np.abs(got - ref)[ok]) < 2e-3
+    assert np.all(got[:, 9] == 3.0)
+
+
+def test_chan_stats_vs_oracle(gpu, orc):
+    vfrb = gpu
+    rng = np.random.default_rng(2)
+    rows, nch, ncol, nblk = 2000, 300, 400, 5
+    fb = rng.normal(0, 10, (rows, nch)).astype(np.float32)
+    fb[:, 3] = 1.0
+    d = vfrb.DeviceArray.from_host(fb)
+    ave, sig, out = vfrb.chan_stats(d, rows, nch, ncol, nblk)
+    rave, rsig, rout = orc.chan_stats(fb, ncol, nblk)
+    assert np.allclose(ave, rave, atol=1e-4) and np.allclose(sig, rsig, rtol=1e-4, atol=1e-4)
+    assert np.array_equal(out, rout)
+
+
+# ------------------------------------------------------- dedispersion ----
+def _c1_shape():
+    nch, tchan = 1024, 64 * 2048 / 128e6
+    chanbw = 64.0 / nch
+    fch1 = 320.0 + nch * chanbw - 0.5 * chanbw
+    dms = (np.arange(256) * 0.1).astype(np.float32)
+    return nch, tchan, fch1, chanbw, dms
+
+
+def test_delay_table_device_matches_oracle(gpu, orc):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, 976)
+    off, maxd = orc.delay_table(dms, nch, fch1, chanbw, tchan)
+    assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
+
+
+@pytest.mark.parametrize("tile", range(8))
+def test_dedisperse_f32_all_tiles(gpu, orc, tile):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(tile)
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    print(f"[case] dedisperse f32 tile={tile} ({vfrb.tile_names()[tile]})", flush=True)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORE_F32, tile)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.execute(rows, d, out, nt)
+    got = out.download()
+    ref = orc.dedisperse_f32(fb, plan.offsets(), nt)
+    # same terms, different summation tree: f32 rounding only
+    assert np.max(np.abs(got - ref)) < 1e-4 * 22
+
+
+def test_load_bandpass_then_dedisperse_matches_oracle_chain(gpu, orc):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(4)
+    fb = rng.normal(256, 22, (rows, nch)).astype(np.float32)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, bandpass=True)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.dedisperse(rows, out, nt)
+    got = out.download()
+    bp = orc.bandpass_f32(fb)
+    ref = orc.dedisperse_f32(bp, plan.offsets(), nt)
+    assert np.max(np.abs(got - ref)) < 2e-3
+
+
+def test_load_from_ring(gpu, orc):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows, ring, row0 = 700, 1000, 800
+    rng = np.random.default_rng(6)
+    fb = rng.normal(256, 22, (ring, nch)).astype(np.float32)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, ring_rows=ring, row0=row0, bandpass=True)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.dedisperse(rows, out, nt)
+    window = fb[[(row0 + r) % ring for r in range(rows)]]
+    ref = orc.dedisperse_f32(orc.bandpass_f32(window), plan.offsets(), nt)
+    assert np.max(np.abs(out.download() - ref)) < 2e-3
+
+
+@pytest.mark.parametrize("storage,tol", [("f16", 2e-3), ("u8", 2.5e-2)])
+def test_reduced_precision_storage(gpu, orc, storage, tol):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(8)
+    fb = rng.normal(256, 22, (rows, nch)).astype(np.float32)
+    fb[:, 40] += 3.0e6            # strong RFI channel: exercises the scale logic
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[storage])
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, bandpass=True)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.dedisperse(rows, out, nt)
+    got = out.download()
+    ref = orc.dedisperse_f64(orc.bandpass_f64(fb.astype(np.float64)), plan.offsets(), nt)
+    sd = ref.std()
+    # error relative to the dedispersed noise level
+    assert np.max(np.abs(got - ref)) / sd < tol, np.max(np.abs(got - ref)) / sd
+
+
+def test_dedisperse_stats_variant(gpu, orc):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw = 3840, 4 * 12500 / 128e6, 64.0 / 6250, None
+    chanbw = 64.0 / 6250
+    fch1 = 320.0 + nch * chanbw - 0.5 * chanbw
+    dms = np.array([0.0, 119.3], np.float32)
+    off, maxd = orc.delay_table(dms, nch, fch1, chanbw, tchan)
+    rows = maxd + 500
+    rng = np.random.default_rng(3)
+    fb = rng.normal(0, 5, (rows, nch)).astype(np.float32)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, bandpass=False)
+    out, sig = vfrb.DeviceArray((2, nt), np.float32), vfrb.DeviceArray((2, nt), np.float32)
+    plan.dedisperse_stats(rows, out, sig, nt)
+    rout, rsig = orc.dedisperse_stats_f32(fb, off, nt)
+    assert np.allclose(out.download(), rout, atol=1e-4)
+    assert np.allclose(sig.download(), rsig, rtol=1e-3)
+
+
+# --------------------------------------------------------- statistics ----
+def test_dm_stats_f64_matches_reference_arithmetic(gpu, orc):
+    vfrb = gpu
+    rng = np.random.default_rng(10)
+    ndm, nt = 64, 19531
+    x = rng.normal(3.0, 0.7, (ndm, nt)).astype(np.float32)
+    x[5, 1234] += 9.0
+    x[7] = 2.0  # constant series: sigma 0
+    d = vfrb.DeviceArray.from_host(x)
+    st = vfrb.dm_stats(d, nt, ndm, nt, vfrb.ACC_F64)
+    frb, ave, sig, peak, pt = orc.dm_stats(x)
+    ok = np.arange(ndm) != 7
+    # FP64 sums of the same floats: the float results agree to the last bit
+    # except where the double sum sits on a float rounding boundary.
+    assert np.max(np.abs(st.ave - ave)[ok] / np.abs(ave[ok])) < 2e-7
+    assert np.max(np.abs(st.sigma - sig)[ok] / sig[ok]) < 2e-7
+    assert np.array_equal(st.frb, frb) and np.array_equal(st.peak_t, pt)
+    assert np.allclose(st.peak, peak, rtol=1e-6)
+    assert st.peak_t[5] == 1234
+    stc = vfrb.dm_stats(d, nt, ndm, nt, vfrb.ACC_F32C)
+    assert np.max(np.abs(stc.sigma - sig)[ok] / sig[ok]) < 1e-5
+    assert np.array_equal(stc.peak_t[ok], pt[ok])
+
+
+def test_candidates(gpu):
+    vfrb = gpu
+    rng = np.random.default_rng(12)
+    ndm, nt = 32, 5000
+    x = rng.normal(0, 1, (ndm, nt)).astype(np.float32)
+    x[3, 100] = 12.0
+    x[20, 4000] = 9.0
+    d = vfrb.DeviceArray.from_host(x)
+    st = vfrb.dm_stats(d, nt, ndm, nt)
+    n, cdm, ct, cs = vfrb.candidates(d, nt, ndm, nt, st.ave, st.sigma, 7.0)
+    snr = (x - st.ave[:, None]) / st.sigma[:, None]
+    want = set(zip(*np.nonzero(snr > 7.0)))
+    assert n == len(want) and set(zip(cdm.tolist(), ct.tolist())) == want

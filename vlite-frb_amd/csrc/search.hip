@@ -1,0 +1,668 @@
+// Back end: window load (bandpass + channel-major pack in the storage type)
+// and brute-force dedispersion over LDS-tiled channel x time blocks.
+//
+// Data layout in HBM
+//   fb ring   : f32 [ring_rows][nch], time-major, written by the front end.
+//   window    : storage type [nch][ld], channel-major residuals of one window.
+//   out       : f32 [ndm][ld_out].
+// The dedispersion of DM d and sample t reads window[c][t + off[d][c]] for
+// all channels c.  A workgroup owns a block of DMs x a tile of T samples;
+// for each chunk of channels it stages window[c][t0+dmin_c .. t0+T+dmax_c)
+// in LDS once and every wave accumulates its DMs from there.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include <hip/hip_fp16.h>
+
+#include "bandpass.hpp"
+#include "common.hpp"
+
+using vfrb::BP_CH;
+using vfrb::BP_PER;
+using vfrb::BP_Q;
+using vfrb::BP_ROWS;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Window load
+// ---------------------------------------------------------------------------
+enum LoadMode { LOAD_F32 = 0, LOAD_STATS = 1, LOAD_APPLY_F16 = 2, LOAD_APPLY_U8 = 3 };
+
+__device__ __forceinline__ unsigned float_order(float f) {
+    const unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float float_unorder(unsigned u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// grid = (ceil(nch/64), ceil(nrows/128)), block = 256.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+load_kernel(const float* __restrict__ fb, int64_t ring_rows, int64_t row0, int nrows, int nch,
+            int apply_bp, void* __restrict__ win, int64_t ld, float* __restrict__ bp_mean,
+            unsigned* __restrict__ range /* [2][nch] ordered min/max, or [1] global max|.| */,
+            const int* __restrict__ f16_exp, const float* __restrict__ u8_scale,
+            const float* __restrict__ u8_off) {
+    __shared__ float red[BP_Q * BP_CH];
+    __shared__ int ired[BP_Q * BP_CH];
+    __shared__ float tile[BP_CH][BP_ROWS + 1];
+    const int lane = threadIdx.x & 63;
+    const int q = threadIdx.x >> 6;
+    const int c = blockIdx.x * BP_CH + lane;
+    const int blk = blockIdx.y;
+    const int rbase = blk * BP_ROWS;
+    const int npts = min(BP_ROWS, nrows - rbase);
+    const int mine0 = q * BP_PER;
+    const int n_mine = max(0, min(BP_PER, npts - mine0));
+    const bool cvalid = c < nch;
+
+    float x[BP_PER];
+#pragma unroll
+    for (int i = 0; i < BP_PER; i++) {
+        x[i] = 0.0f;
+        if (i < n_mine && cvalid) {
+            int64_t r = row0 + rbase + mine0 + i;
+            if (ring_rows > 0) r %= ring_rows;
+            x[i] = fb[r * nch + c];
+        }
+    }
+    float m = 0.0f;
+    if (MODE == LOAD_F32 || MODE == LOAD_STATS) {
+        if (apply_bp) m = vfrb::bp_clipped_mean(x, cvalid ? n_mine : 0, npts, red, ired, lane, q);
+        if (MODE == LOAD_STATS) {
+            if (q == 0 && cvalid) bp_mean[(int64_t)blk * nch + c] = m;
+            if (cvalid && n_mine > 0) {
+                float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+                for (int i = 0; i < BP_PER; i++) {
+                    if (i < n_mine) {
+                        const float y = x[i] - m;
+                        lo = fminf(lo, y);
+                        hi = fmaxf(hi, y);
+                    }
+                }
+                if (range) {
+                    // range[0..nch): ordered min, range[nch..2nch): ordered max
+                    atomicMin(&range[c], float_order(lo));
+                    atomicMax(&range[nch + c], float_order(hi));
+                }
+            }
+            return;
+        }
+    } else {
+        if (apply_bp && cvalid) m = bp_mean[(int64_t)blk * nch + c];
+    }
+    // residuals -> LDS tile [channel][row]
+#pragma unroll
+    for (int i = 0; i < BP_PER; i++) tile[lane][mine0 + i] = x[i] - m;
+    __syncthreads();
+    // channel-major store: each wave writes whole channel rows, 2 rows/lane
+    const int w = threadIdx.x >> 6;
+    const int r0 = 2 * lane;
+    for (int cl = w; cl < BP_CH; cl += 4) {
+        const int cc = blockIdx.x * BP_CH + cl;
+        if (cc >= nch) break;
+        const float v0 = tile[cl][r0], v1 = tile[cl][r0 + 1];
+        const int64_t o = (int64_t)cc * ld + rbase + r0;
+        if (MODE == LOAD_F32) {
+            float* dst = reinterpret_cast<float*>(win) + o;
+            if (r0 + 1 < npts) {
+                *reinterpret_cast<float2*>(dst) = make_float2(v0, v1);
+            } else if (r0 < npts) {
+                dst[0] = v0;
+            }
+        } else if (MODE == LOAD_APPLY_F16) {
+            const float s = ldexpf(1.0f, -f16_exp[0]);
+            __half* dst = reinterpret_cast<__half*>(win) + o;
+            if (r0 + 1 < npts) {
+                *reinterpret_cast<__half2*>(dst) = __floats2half2_rn(v0 * s, v1 * s);
+            } else if (r0 < npts) {
+                dst[0] = __float2half_rn(v0 * s);
+            }
+        } else {  // LOAD_APPLY_U8
+            const float sc = u8_scale[cc], of = u8_off[cc];
+            const float inv = 1.0f / sc;
+            auto qz = [&](float v) {
+                float t = rintf((v - of) * inv);
+                t = fminf(fmaxf(t, 0.0f), 255.0f);
+                return (unsigned)t;
+            };
+            uint8_t* dst = reinterpret_cast<uint8_t*>(win) + o;
+            if (r0 + 1 < npts) {
+                dst[0] = (uint8_t)qz(v0);
+                dst[1] = (uint8_t)qz(v1);
+            } else if (r0 < npts) {
+                dst[0] = (uint8_t)qz(v0);
+            }
+        }
+    }
+}
+
+// Turns the ordered per-channel ranges into the storage scales.
+//  f16: one power-of-two exponent e with max|y| * 2^-e <= 32768.
+//  u8 : per channel offset = min, scale = (max-min)/255 (1 if flat).
+__global__ void finish_range_kernel(const unsigned* __restrict__ range, int nch, int storage,
+                                    int* __restrict__ f16_exp, float* __restrict__ u8_scale,
+                                    float* __restrict__ u8_off, float* __restrict__ u8_offsum) {
+    __shared__ float part[256];
+    __shared__ float partsum[256];
+    float mx = 0.0f, osum = 0.0f;
+    for (int c = threadIdx.x; c < nch; c += blockDim.x) {
+        const float lo = float_unorder(range[c]);
+        const float hi = float_unorder(range[nch + c]);
+        if (storage == VFRB_STORE_F16) {
+            mx = fmaxf(mx, fmaxf(fabsf(lo), fabsf(hi)));
+        } else {
+            float sc = (hi - lo) / 255.0f;
+            if (!(sc > 0.0f) || !isfinite(sc)) sc = 1.0f;
+            const float of = isfinite(lo) ? lo : 0.0f;
+            u8_scale[c] = sc;
+            u8_off[c] = of;
+            osum += of;
+        }
+    }
+    part[threadIdx.x] = mx;
+    partsum[threadIdx.x] = osum;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            part[threadIdx.x] = fmaxf(part[threadIdx.x], part[threadIdx.x + s]);
+            partsum[threadIdx.x] += partsum[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (storage == VFRB_STORE_F16) {
+            int e = 0;
+            const float m = part[0];
+            if (isfinite(m) && m > 32768.0f) e = (int)ceilf(log2f(m / 32768.0f));
+            f16_exp[0] = e;
+        } else {
+            u8_offsum[0] = partsum[0];
+        }
+    }
+}
+
+__global__ void range_init_kernel(unsigned* range, int nch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nch) range[i] = 0xffffffffu;      // ordered +max -> atomicMin start
+    else if (i < 2 * nch) range[i] = 0u;      // ordered -max -> atomicMax start
+}
+
+// ---------------------------------------------------------------------------
+// Dedispersion
+// ---------------------------------------------------------------------------
+template <typename TS>
+__device__ __forceinline__ float to_f32(TS v);
+template <>
+__device__ __forceinline__ float to_f32<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ float to_f32<__half>(__half v) { return __half2float(v); }
+template <>
+__device__ __forceinline__ float to_f32<uint8_t>(uint8_t v) { return (float)v; }
+
+struct Chunk {
+    int c0, count, stride, pad;
+};
+
+// SL samples per lane, DW DMs per wave, W waves; T = 64*SL samples per tile.
+template <typename TS, int SL, int DW, int W>
+__global__ void __launch_bounds__(64 * W)
+dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+              const uint16_t* __restrict__ rel,  // [nblk][nch][W*DW]
+              const int* __restrict__ dmin,      // [nblk][nch]
+              const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+              const float* __restrict__ chan_scale,  // u8 only
+              const int* __restrict__ f16_exp, const float* __restrict__ offsum,
+              float* __restrict__ out, int64_t ld_out) {
+    extern __shared__ unsigned char lds_raw[];
+    TS* lds = reinterpret_cast<TS*>(lds_raw);
+    constexpr int T = 64 * SL;
+    constexpr int DMB = W * DW;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int b = blockIdx.y;
+    const int t0 = blockIdx.x * T;
+    const int dm0 = b * DMB + wave * DW;
+    const int2 cr = blk_chunks[b];
+
+    float acc[DW][SL];
+#pragma unroll
+    for (int d = 0; d < DW; d++)
+#pragma unroll
+        for (int j = 0; j < SL; j++) acc[d][j] = 0.0f;
+
+    for (int k = 0; k < cr.y; k++) {
+        const Chunk ch = chunks[cr.x + k];
+        // stage window[c][t0 + dmin_c + s], s < stride, rows >= nrows -> 0
+        for (int ci = wave; ci < ch.count; ci += W) {
+            const int c = ch.c0 + ci;
+            const int64_t src0 = (int64_t)c * ld;
+            const int r0 = t0 + dmin[(int64_t)b * nch + c];
+            TS* dst = lds + ci * ch.stride;
+            for (int s = lane; s < ch.stride; s += 64) {
+                const int r = r0 + s;
+                dst[s] = r < nrows ? win[src0 + r] : TS(0);
+            }
+        }
+        __syncthreads();
+        for (int ci = 0; ci < ch.count; ci++) {
+            const int c = ch.c0 + ci;
+            const uint16_t* rp = rel + ((int64_t)b * nch + c) * DMB + wave * DW;
+            const TS* row = lds + ci * ch.stride + lane;
+            float sc = 1.0f;
+            if constexpr (sizeof(TS) == 1) sc = chan_scale[c];
+#pragma unroll
+            for (int d = 0; d < DW; d++) {
+                const TS* p = row + rp[d];
+#pragma unroll
+                for (int j = 0; j < SL; j++) {
+                    if constexpr (sizeof(TS) == 1) acc[d][j] = fmaf(sc, to_f32(p[64 * j]), acc[d][j]);
+                    else acc[d][j] += to_f32(p[64 * j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // f32: sum / nch, the reference's final division (PIPELINE:1535);
+    // f16: undo the power-of-two storage scale; u8: add the channel offsets.
+    const float fnch = (float)nch;
+    float mul = 1.0f, add = 0.0f;
+    if constexpr (sizeof(TS) == 2) mul = ldexpf(1.0f, f16_exp[0]);
+    if constexpr (sizeof(TS) == 1) add = offsum[0];
+#pragma unroll
+    for (int d = 0; d < DW; d++) {
+        const int dm = dm0 + d;
+        if (dm >= ndm) break;
+        float* o = out + (int64_t)dm * ld_out;
+#pragma unroll
+        for (int j = 0; j < SL; j++) {
+            const int t = t0 + lane + 64 * j;
+            if (t < nt) {
+                if constexpr (sizeof(TS) == 4) o[t] = acc[d][j] / fnch;
+                else if constexpr (sizeof(TS) == 2) o[t] = (acc[d][j] * mul) / fnch;
+                else o[t] = (acc[d][j] + add) / fnch;
+            }
+        }
+    }
+}
+
+// dedisperseSTATS (TIMESERIES:1450-1481), one thread per (dm, t), f32 window.
+__global__ void dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, int nch, int nt,
+                                    int ndm, const int* __restrict__ off, float* __restrict__ out,
+                                    float* __restrict__ sig, int64_t ld_out) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)nt * ndm) return;
+    const int dm = (int)(idx / nt);
+    const int t = (int)(idx - (int64_t)dm * nt);
+    const int* o = off + (int64_t)dm * nch;
+    float sum = 0.0f;
+    for (int c = 0; c < nch; c++) sum += win[(int64_t)c * ld + t + o[c]];
+    const float ave = sum / (float)nch;
+    float s1 = 0.0f, s2 = 0.0f;
+    for (int c = 0; c < nch; c++) {
+        const float d = win[(int64_t)c * ld + t + o[c]] - ave;
+        s1 += d;
+        s2 += d * d;
+    }
+    const float sigma = (s2 - (s1 * s1)) / (float)(nch * (nch - 1));
+    out[(int64_t)dm * ld_out + t] = ave;
+    sig[(int64_t)dm * ld_out + t] = sqrtf(sigma);
+}
+
+// ---------------------------------------------------------------------------
+// Tile configurations
+// ---------------------------------------------------------------------------
+struct TileCfg {
+    int sl, dw, w;
+    const char* name;
+};
+constexpr TileCfg kTiles[] = {
+    {4, 8, 4, "T256_dm32_w4"},  {4, 16, 4, "T256_dm64_w4"}, {8, 8, 4, "T512_dm32_w4"},
+    {2, 16, 4, "T128_dm64_w4"}, {4, 8, 8, "T256_dm64_w8"},  {8, 16, 4, "T512_dm64_w4"},
+    {2, 8, 8, "T128_dm64_w8"},  {4, 4, 8, "T256_dm32_w8"},
+};
+constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
+constexpr int kDefaultTile = 1;
+constexpr size_t kLdsBudget = 64 * 1024;
+
+template <typename TS, int I>
+const void* tile_kernel_ptr() {
+    return reinterpret_cast<const void*>(
+        &dedisp_kernel<TS, kTiles[I].sl, kTiles[I].dw, kTiles[I].w>);
+}
+template <typename TS>
+const void* tile_kernel(int i) {
+    switch (i) {
+        case 0: return tile_kernel_ptr<TS, 0>();
+        case 1: return tile_kernel_ptr<TS, 1>();
+        case 2: return tile_kernel_ptr<TS, 2>();
+        case 3: return tile_kernel_ptr<TS, 3>();
+        case 4: return tile_kernel_ptr<TS, 4>();
+        case 5: return tile_kernel_ptr<TS, 5>();
+        case 6: return tile_kernel_ptr<TS, 6>();
+        default: return tile_kernel_ptr<TS, 7>();
+    }
+}
+
+size_t storage_bytes(int storage) {
+    return storage == VFRB_STORE_F32 ? 4 : storage == VFRB_STORE_F16 ? 2 : 1;
+}
+
+}  // namespace
+
+struct vfrb_search_plan {
+    vfrb_search_desc desc{};
+    std::vector<float> dms;
+    std::vector<int32_t> off;  // host [ndm][nch]
+    int32_t max_delay = 0;
+    int nch = 0, ndm = 0, storage = 0;
+    int64_t ld = 0;            // window row length (elements)
+    int tile = -1;
+    // device
+    int32_t* d_off = nullptr;
+    void* d_win = nullptr;
+    float* d_bp_mean = nullptr;
+    unsigned* d_range = nullptr;
+    int* d_f16_exp = nullptr;
+    float* d_u8_scale = nullptr;
+    float* d_u8_off = nullptr;
+    float* d_u8_offsum = nullptr;
+    float* d_tmp_fb = nullptr;  // time-major staging for vfrb_search_execute
+    // tile tables
+    uint16_t* d_rel = nullptr;
+    int* d_dmin = nullptr;
+    Chunk* d_chunks = nullptr;
+    int2* d_blk_chunks = nullptr;
+    int nblk = 0;
+    size_t lds_bytes = 0;
+    int loaded_rows = 0;
+};
+
+namespace {
+
+void free_tiles(vfrb_search_plan* p) {
+    (void)hipFree(p->d_rel);
+    (void)hipFree(p->d_dmin);
+    (void)hipFree(p->d_chunks);
+    (void)hipFree(p->d_blk_chunks);
+    p->d_rel = nullptr;
+    p->d_dmin = nullptr;
+    p->d_chunks = nullptr;
+    p->d_blk_chunks = nullptr;
+}
+
+// Builds the per-DM-block tables for tile configuration i: minimum delay per
+// channel, delays relative to it (u16), and channel chunks whose staged
+// spans fit the LDS budget.
+vfrb_status build_tiles(vfrb_search_plan* p, int i) {
+    const TileCfg& tc = kTiles[i];
+    const int T = 64 * tc.sl, DMB = tc.w * tc.dw;
+    const int nch = p->nch, ndm = p->ndm;
+    const int nblk = (ndm + DMB - 1) / DMB;
+    const size_t esz = storage_bytes(p->storage);
+    std::vector<uint16_t> rel((size_t)nblk * nch * DMB, 0);
+    std::vector<int> dmin((size_t)nblk * nch, 0);
+    std::vector<Chunk> chunks;
+    std::vector<int2> blk(nblk);
+    for (int b = 0; b < nblk; b++) {
+        const int d0 = b * DMB, d1 = std::min(ndm, d0 + DMB);
+        std::vector<int> span(nch);
+        for (int c = 0; c < nch; c++) {
+            int lo = INT32_MAX, hi = INT32_MIN;
+            for (int d = d0; d < d1; d++) {
+                lo = std::min(lo, p->off[(size_t)d * nch + c]);
+                hi = std::max(hi, p->off[(size_t)d * nch + c]);
+            }
+            VFRB_REQUIRE(hi - lo < 65536, VFRB_ERR_UNSUPPORTED,
+                         "delay spread %d in a DM block exceeds 65535 rows", hi - lo);
+            dmin[(size_t)b * nch + c] = lo;
+            span[c] = hi - lo;
+            for (int d = d0; d < d1; d++)
+                rel[((size_t)b * nch + c) * DMB + (d - d0)] = (uint16_t)(p->off[(size_t)d * nch + c] - lo);
+        }
+        blk[b].x = (int)chunks.size();
+        int c = 0;
+        while (c < nch) {
+            int count = 0, smax = 0;
+            while (c + count < nch) {
+                const int s2 = std::max(smax, span[c + count]);
+                const size_t need = (size_t)(count + 1) * (size_t)(T + s2) * esz;
+                if (need > kLdsBudget || count >= 64) break;
+                smax = s2;
+                count++;
+            }
+            VFRB_REQUIRE(count > 0, VFRB_ERR_UNSUPPORTED,
+                         "one channel's staged span (%d rows) exceeds the LDS budget; use a tile "
+                         "with fewer DMs per block", T + span[c]);
+            chunks.push_back(Chunk{c, count, T + smax, 0});
+            c += count;
+        }
+        blk[b].y = (int)chunks.size() - blk[b].x;
+    }
+    free_tiles(p);
+    hipError_t e = hipMalloc(&p->d_rel, rel.size() * sizeof(uint16_t));
+    if (e == hipSuccess) e = hipMalloc(&p->d_dmin, dmin.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_chunks, chunks.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc(&p->d_blk_chunks, blk.size() * sizeof(int2));
+    if (e == hipSuccess) e = hipMemcpy(p->d_rel, rel.data(), rel.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_dmin, dmin.data(), dmin.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_blk_chunks, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_tiles(p);
+        return vfrb::hip_fail(e, "build_tiles", __FILE__, __LINE__);
+    }
+    p->nblk = nblk;
+    p->tile = i;
+    p->lds_bytes = kLdsBudget;
+    return VFRB_OK;
+}
+
+const void* dedisp_kernel_for(const vfrb_search_plan* p) {
+    if (p->storage == VFRB_STORE_F32) return tile_kernel<float>(p->tile);
+    if (p->storage == VFRB_STORE_F16) return tile_kernel<__half>(p->tile);
+    return tile_kernel<uint8_t>(p->tile);
+}
+
+void destroy(vfrb_search_plan* p) {
+    free_tiles(p);
+    (void)hipFree(p->d_off);
+    (void)hipFree(p->d_win);
+    (void)hipFree(p->d_bp_mean);
+    (void)hipFree(p->d_range);
+    (void)hipFree(p->d_f16_exp);
+    (void)hipFree(p->d_u8_scale);
+    (void)hipFree(p->d_u8_off);
+    (void)hipFree(p->d_u8_offsum);
+    (void)hipFree(p->d_tmp_fb);
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vfrb_search_tile_count(void) { return kNumTiles; }
+
+const char* vfrb_search_tile_name(int32_t i) {
+    if (i < 0 || i >= kNumTiles) return "";
+    return kTiles[i].name;
+}
+
+vfrb_status vfrb_search_plan_create(vfrb_search_plan** plan, const vfrb_search_desc* d) {
+    VFRB_REQUIRE(plan && d, VFRB_ERR_INVALID, "NULL argument");
+    *plan = nullptr;
+    VFRB_REQUIRE(d->nchans > 0 && d->ndm > 0 && d->dms, VFRB_ERR_INVALID, "empty channel or DM set");
+    VFRB_REQUIRE(d->max_rows > 0, VFRB_ERR_INVALID, "max_rows=%d", d->max_rows);
+    VFRB_REQUIRE(d->storage >= VFRB_STORE_F32 && d->storage <= VFRB_STORE_U8, VFRB_ERR_INVALID,
+                 "storage=%d", d->storage);
+    VFRB_REQUIRE(d->tile_config >= -1 && d->tile_config < kNumTiles, VFRB_ERR_INVALID,
+                 "tile_config=%d", d->tile_config);
+    vfrb_search_plan* p = new vfrb_search_plan();
+    p->desc = *d;
+    p->nch = d->nchans;
+    p->ndm = d->ndm;
+    p->storage = d->storage;
+    p->dms.assign(d->dms, d->dms + d->ndm);
+    p->desc.dms = p->dms.data();
+    p->off.resize((size_t)p->ndm * p->nch);
+    vfrb_status st = vfrb_delay_table(p->dms.data(), p->ndm, p->nch, d->fch1, d->chanbw, d->tsamp,
+                                      p->off.data(), &p->max_delay, nullptr);
+    if (st != VFRB_OK) {
+        delete p;
+        return st;
+    }
+    int32_t maxoff = 0;
+    for (int32_t v : p->off) maxoff = std::max(maxoff, v);
+    if (maxoff > p->max_delay) {
+        delete p;
+        vfrb::set_error("largest delay %d exceeds max_delay %d of the last DM (DM list must end "
+                        "with its largest value)", maxoff, p->max_delay);
+        return VFRB_ERR_INVALID;
+    }
+    // Window rows padded to a multiple of 64 elements (256-byte rows in f32).
+    p->ld = ((int64_t)d->max_rows + 63) / 64 * 64;
+    const size_t win_bytes = (size_t)p->nch * p->ld * storage_bytes(p->storage);
+    const int nbpblk = (d->max_rows + BP_ROWS - 1) / BP_ROWS;
+    hipError_t e = hipMalloc(&p->d_off, p->off.size() * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMemcpy(p->d_off, p->off.data(), p->off.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&p->d_win, win_bytes);
+    if (e == hipSuccess) e = hipMemset(p->d_win, 0, win_bytes);
+    if (e == hipSuccess) e = hipMalloc(&p->d_bp_mean, sizeof(float) * (size_t)nbpblk * p->nch);
+    if (e == hipSuccess) e = hipMalloc(&p->d_range, sizeof(unsigned) * 2 * p->nch);
+    if (e == hipSuccess) e = hipMalloc(&p->d_f16_exp, sizeof(int));
+    if (e == hipSuccess) e = hipMemset(p->d_f16_exp, 0, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&p->d_u8_scale, sizeof(float) * p->nch);
+    if (e == hipSuccess) e = hipMalloc(&p->d_u8_off, sizeof(float) * p->nch);
+    if (e == hipSuccess) e = hipMalloc(&p->d_u8_offsum, sizeof(float));
+    if (e != hipSuccess) {
+        st = vfrb::hip_fail(e, "vfrb_search_plan_create", __FILE__, __LINE__);
+        destroy(p);
+        return st;
+    }
+    st = build_tiles(p, d->tile_config >= 0 ? d->tile_config : kDefaultTile);
+    if (st != VFRB_OK) {
+        destroy(p);
+        return st;
+    }
+    *plan = p;
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_search_plan_destroy(vfrb_search_plan* p) {
+    if (p) destroy(p);
+    return VFRB_OK;
+}
+
+int32_t vfrb_search_max_delay(const vfrb_search_plan* p) { return p ? p->max_delay : -1; }
+int32_t vfrb_search_dm_count(const vfrb_search_plan* p) { return p ? p->ndm : -1; }
+
+vfrb_status vfrb_search_get_offsets(const vfrb_search_plan* p, int32_t* host) {
+    VFRB_REQUIRE(p && host, VFRB_ERR_INVALID, "NULL argument");
+    std::memcpy(host, p->off.data(), p->off.size() * sizeof(int32_t));
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_search_set_tile(vfrb_search_plan* p, int32_t i) {
+    VFRB_REQUIRE(p && i >= 0 && i < kNumTiles, VFRB_ERR_INVALID, "tile %d", i);
+    if (p->tile == i) return VFRB_OK;
+    return build_tiles(p, i);
+}
+
+int32_t vfrb_search_get_tile(const vfrb_search_plan* p) { return p ? p->tile : -1; }
+
+vfrb_status vfrb_search_load(vfrb_search_plan* p, const float* fb, int64_t ring_rows, int64_t row0,
+                             int32_t nrows, int32_t apply_bp, void* stream) {
+    VFRB_REQUIRE(p && fb, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(nrows > 0 && nrows <= p->desc.max_rows, VFRB_ERR_INVALID,
+                 "nrows=%d outside 1..%d", nrows, p->desc.max_rows);
+    VFRB_REQUIRE(ring_rows == 0 || nrows <= ring_rows, VFRB_ERR_INVALID, "window exceeds ring");
+    VFRB_REQUIRE(row0 >= 0, VFRB_ERR_INVALID, "row0 < 0");
+    hipStream_t s = vfrb::as_stream(stream);
+    dim3 grid((unsigned)vfrb::ceil_div(p->nch, BP_CH), (unsigned)vfrb::ceil_div(nrows, BP_ROWS));
+    dim3 block(256);
+    if (p->storage == VFRB_STORE_F32) {
+        hipLaunchKernelGGL(load_kernel<LOAD_F32>, grid, block, 0, s, fb, ring_rows, row0, nrows,
+                           p->nch, apply_bp, p->d_win, p->ld, p->d_bp_mean, nullptr, nullptr,
+                           nullptr, nullptr);
+        VFRB_LAUNCH_CHECK("load_kernel<f32>");
+    } else {
+        hipLaunchKernelGGL(range_init_kernel, dim3((unsigned)vfrb::ceil_div(2 * p->nch, 256)), dim3(256),
+                           0, s, p->d_range, p->nch);
+        VFRB_LAUNCH_CHECK("range_init_kernel");
+        hipLaunchKernelGGL(load_kernel<LOAD_STATS>, grid, block, 0, s, fb, ring_rows, row0, nrows,
+                           p->nch, apply_bp, p->d_win, p->ld, p->d_bp_mean, p->d_range, nullptr,
+                           nullptr, nullptr);
+        VFRB_LAUNCH_CHECK("load_kernel<stats>");
+        hipLaunchKernelGGL(finish_range_kernel, dim3(1), dim3(256), 0, s, p->d_range, p->nch,
+                           p->storage, p->d_f16_exp, p->d_u8_scale, p->d_u8_off, p->d_u8_offsum);
+        VFRB_LAUNCH_CHECK("finish_range_kernel");
+        if (p->storage == VFRB_STORE_F16) {
+            hipLaunchKernelGGL(load_kernel<LOAD_APPLY_F16>, grid, block, 0, s, fb, ring_rows, row0,
+                               nrows, p->nch, apply_bp, p->d_win, p->ld, p->d_bp_mean, nullptr,
+                               p->d_f16_exp, nullptr, nullptr);
+        } else {
+            hipLaunchKernelGGL(load_kernel<LOAD_APPLY_U8>, grid, block, 0, s, fb, ring_rows, row0,
+                               nrows, p->nch, apply_bp, p->d_win, p->ld, p->d_bp_mean, nullptr,
+                               nullptr, p->d_u8_scale, p->d_u8_off);
+        }
+        VFRB_LAUNCH_CHECK("load_kernel<apply>");
+    }
+    p->loaded_rows = nrows;
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* out, int64_t ld_out,
+                                   void* stream) {
+    VFRB_REQUIRE(p && out, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(nrows > 0 && nrows <= p->desc.max_rows, VFRB_ERR_INVALID, "nrows=%d", nrows);
+    int nt = nrows - p->max_delay;
+    VFRB_REQUIRE(nt > 0, VFRB_ERR_INVALID, "window of %d rows is not longer than max_delay %d",
+                 nrows, p->max_delay);
+    VFRB_REQUIRE(ld_out >= nt, VFRB_ERR_INVALID, "ld_out=%lld < %d", (long long)ld_out, nt);
+    VFRB_REQUIRE(nrows <= p->loaded_rows, VFRB_ERR_INVALID,
+                 "window holds %d rows, %d requested", p->loaded_rows, nrows);
+    const TileCfg& tc = kTiles[p->tile];
+    const int T = 64 * tc.sl;
+    dim3 grid((unsigned)vfrb::ceil_div(nt, T), (unsigned)p->nblk), block(64 * tc.w);
+    const void* kern = dedisp_kernel_for(p);
+    VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes));
+    const void* win = p->d_win;
+    int64_t ld = p->ld;
+    int nch = p->nch, ndm = p->ndm;
+    void* args[] = {(void*)&win, (void*)&ld, (void*)&nch, (void*)&nrows, (void*)&nt,
+                    (void*)&ndm, (void*)&p->d_rel, (void*)&p->d_dmin, (void*)&p->d_chunks,
+                    (void*)&p->d_blk_chunks, (void*)&p->d_u8_scale, (void*)&p->d_f16_exp,
+                    (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out};
+    VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->lds_bytes, vfrb::as_stream(stream)));
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_search_execute(vfrb_search_plan* p, int32_t nrows, const float* fb_tm, float* out,
+                                int64_t ld_out, void* stream) {
+    vfrb_status st = vfrb_search_load(p, fb_tm, 0, 0, nrows, 0, stream);
+    if (st != VFRB_OK) return st;
+    return vfrb_search_dedisperse(p, nrows, out, ld_out, stream);
+}
+
+vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* p, int32_t nrows, float* out, float* sig,
+                                         int64_t ld_out, void* stream) {
+    VFRB_REQUIRE(p && out && sig, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(p->storage == VFRB_STORE_F32, VFRB_ERR_UNSUPPORTED, "needs an f32 plan");
+    const int nt = nrows - p->max_delay;
+    VFRB_REQUIRE(nrows <= p->desc.max_rows && nt > 0 && ld_out >= nt, VFRB_ERR_INVALID,
+                 "nrows=%d max_delay=%d ld_out=%lld", nrows, p->max_delay, (long long)ld_out);
+    const int64_t total = (int64_t)nt * p->ndm;
+    hipLaunchKernelGGL(dedisp_stats_kernel, dim3((unsigned)vfrb::ceil_div(total, 256)), dim3(256), 0,
+                       vfrb::as_stream(stream), (const float*)p->d_win, p->ld, p->nch, nt, p->ndm,
+                       (const int*)p->d_off, out, sig, ld_out);
+    VFRB_LAUNCH_CHECK("dedisp_stats_kernel");
+    return VFRB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,504 @@
+"""ctypes binding of libvfrb.so (the C ABI declared in include/vfrb.h).
+
+Used by the Python tests, the benchmark and the smoke entry point.  The
+library is loaded from ``_build/libvfrb.so`` next to this file; importing
+this module fails loudly when it has not been built, and every call raises
+VfrbError on a non-zero status (there is no fallback path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libvfrb.so")
+
+STORE_F32, STORE_F16, STORE_U8 = 0, 1, 2
+ACC_F64, ACC_F32C = 0, 1
+STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "u8": STORE_U8}
+
+
+class VfrbError(RuntimeError):
+    def __init__(self, func: str, status: int, message: str):
+        super().__init__(f"{func} failed with status {status}: {message}")
+        self.status = status
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libvfrb.so not built at {LIB_PATH}; run `make -C vlite-frb_amd` "
+                      "or __graft_entry__.build()")
+lib = C.CDLL(LIB_PATH)
+
+vp = C.c_void_p
+i32, i64, f32, f64, u64 = C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_uint64
+sz = C.c_size_t
+P = C.POINTER
+
+
+class Params(C.Structure):
+    _fields_ = [(n, i32) for n in ("twonchan", "nchan", "nfreq", "nave", "nbundle", "nloop",
+                                   "nchanout", "nskip", "maxnt", "nhalfcyc")] + \
+               [(n, f64) for n in ("chanbw", "foff", "fch1", "tchan", "tbundle", "maxtime")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SearchDesc(C.Structure):
+    _fields_ = [("nchans", i32), ("tsamp", f64), ("fch1", f64), ("chanbw", f64), ("ndm", i32),
+                ("dms", P(f32)), ("max_rows", i32), ("storage", i32), ("tile_config", i32)]
+
+
+class VdifHeader(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "seconds", "legacymode", "invalid", "frame", "epoch", "unassigned", "framelength8", "nchan",
+        "version", "stationid", "threadid", "nbits", "iscomplex", "eversion", "extended1",
+        "extended2", "extended3", "extended4")]
+
+
+MAXP, MAXT = 64, 64
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("seed", u64), ("nfft", i32), ("mean", f64), ("sigma", f64), ("f_top_mhz", f64),
+                ("bw_mhz", f64), ("npulse", i32), ("pulse_dm", f64 * MAXP), ("pulse_t0", f64 * MAXP),
+                ("pulse_fwhm", f64 * MAXP), ("pulse_amp", f64 * MAXP), ("ntone", i32),
+                ("tone_bin", f64 * MAXT), ("tone_amp", f64 * MAXT), ("nthreads", i32)]
+
+
+LOG_FN = C.CFUNCTYPE(None, C.c_char_p, vp)
+
+_SIGS = {
+    "vfrb_strerror": (C.c_char_p, [i32]),
+    "vfrb_last_error": (C.c_char_p, []),
+    "vfrb_version": (C.c_char_p, []),
+    "vfrb_params_derive": (i32, [i32, i32, i32, f64, i32, i32, P(Params)]),
+    "vfrb_delay_table": (i32, [P(f32), i32, i32, f64, f64, f64, P(i32), P(i32), P(f64)]),
+    "vfrb_device_count": (i32, [P(i32)]),
+    "vfrb_set_device": (i32, [i32]),
+    "vfrb_device_name": (i32, [C.c_char_p, i32]),
+    "vfrb_mem_info": (i32, [P(sz), P(sz)]),
+    "vfrb_malloc": (i32, [P(vp), sz]),
+    "vfrb_free": (i32, [vp]),
+    "vfrb_host_alloc": (i32, [P(vp), sz]),
+    "vfrb_host_free": (i32, [vp]),
+    "vfrb_memcpy_h2d": (i32, [vp, vp, sz, vp]),
+    "vfrb_memcpy_d2h": (i32, [vp, vp, sz, vp]),
+    "vfrb_memcpy_d2d": (i32, [vp, vp, sz, vp]),
+    "vfrb_memset": (i32, [vp, i32, sz, vp]),
+    "vfrb_stream_create": (i32, [P(vp)]),
+    "vfrb_stream_destroy": (i32, [vp]),
+    "vfrb_stream_synchronize": (i32, [vp]),
+    "vfrb_device_synchronize": (i32, []),
+    "vfrb_event_create": (i32, [P(vp)]),
+    "vfrb_event_destroy": (i32, [vp]),
+    "vfrb_event_record": (i32, [vp, vp]),
+    "vfrb_event_synchronize": (i32, [vp]),
+    "vfrb_event_elapsed_ms": (i32, [P(f32), vp, vp]),
+    "vfrb_fb_plan_create": (i32, [P(vp), i32, i32, i32, i32]),
+    "vfrb_fb_plan_destroy": (i32, [vp]),
+    "vfrb_fb_execute": (i32, [vp, vp, vp, i64, vp, i64, i64, vp]),
+    "vfrb_bandpass_inplace": (i32, [vp, i32, i32, vp]),
+    "vfrb_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+    "vfrb_search_plan_create": (i32, [P(vp), P(SearchDesc)]),
+    "vfrb_search_plan_destroy": (i32, [vp]),
+    "vfrb_search_max_delay": (i32, [vp]),
+    "vfrb_search_dm_count": (i32, [vp]),
+    "vfrb_search_get_offsets": (i32, [vp, P(i32)]),
+    "vfrb_search_load": (i32, [vp, vp, i64, i64, i32, i32, vp]),
+    "vfrb_search_dedisperse": (i32, [vp, i32, vp, i64, vp]),
+    "vfrb_search_execute": (i32, [vp, i32, vp, vp, i64, vp]),
+    "vfrb_search_dedisperse_stats": (i32, [vp, i32, vp, vp, i64, vp]),
+    "vfrb_search_tile_count": (i32, []),
+    "vfrb_search_tile_name": (C.c_char_p, [i32]),
+    "vfrb_search_set_tile": (i32, [vp, i32]),
+    "vfrb_search_get_tile": (i32, [vp]),
+    "vfrb_dm_stats": (i32, [vp, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+    "vfrb_candidates": (i32, [vp, i64, i32, i32, vp, vp, f32, vp, vp, vp, i32, vp, vp]),
+    "vfrb_vdif_decode": (i32, [P(C.c_uint8), P(VdifHeader)]),
+    "vfrb_vdif_encode": (i32, [P(VdifHeader), P(C.c_uint8)]),
+    "vfrb_vdif_dmjd": (f64, [P(C.c_uint8), i32]),
+    "vfrb_seq_open_file": (i32, [P(vp), C.c_char_p]),
+    "vfrb_seq_open_memory": (i32, [P(vp), P(C.c_uint8), sz]),
+    "vfrb_seq_set_log": (i32, [vp, LOG_FN, vp]),
+    "vfrb_seq_init": (i32, [vp]),
+    "vfrb_seq_header": (i32, [vp, P(C.c_uint8)]),
+    "vfrb_seq_fill": (i32, [vp, i32, P(C.c_uint8), P(C.c_uint8)]),
+    "vfrb_seq_counters": (i32, [vp, P(i64), P(i64)]),
+    "vfrb_seq_close": (i32, [vp]),
+    "vfrb_sigproc_header": (i32, [P(C.c_uint8), i32, P(i32), f64, f64, i32, i32, f64, f64]),
+    "vfrb_presto_inf": (i32, [C.c_char_p, i32, C.c_char_p, f64, i32, f64, f32, f64, f64, i32, f64]),
+    "vfrb_synth_payload": (i32, [P(SynthSpec), i32, i64, i64, P(C.c_uint8)]),
+    "vfrb_synth_amp_for_snr": (f64, [f64, i32, i32]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = sorted(_SIGS)
+
+
+def check(status: int, func: str = "vfrb"):
+    if status != 0:
+        msg = lib.vfrb_last_error().decode(errors="replace")
+        raise VfrbError(func, status, msg or lib.vfrb_strerror(status).decode())
+
+
+def call(name, *args):
+    check(getattr(lib, name)(*args), name)
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(P(ctype))
+
+
+# --------------------------------------------------------------------------
+# Host-side helpers
+# --------------------------------------------------------------------------
+def params_derive(twonchan, nave, nbundle, maxtime, fullband=False, nchanout_override=0) -> Params:
+    p = Params()
+    call("vfrb_params_derive", twonchan, nave, nbundle, float(maxtime), int(bool(fullband)),
+         int(nchanout_override), C.byref(p))
+    return p
+
+
+def delay_table(dms, nchanout, fch1, chanbw, tchan):
+    dms = np.ascontiguousarray(dms, dtype=np.float32)
+    off = np.empty((len(dms), nchanout), dtype=np.int32)
+    maxd = i32()
+    margin = f64()
+    call("vfrb_delay_table", _ptr(dms, f32), len(dms), nchanout, fch1, chanbw, tchan,
+         _ptr(off, i32), C.byref(maxd), C.byref(margin))
+    return off, maxd.value, margin.value
+
+
+def vdif_decode(raw: bytes) -> dict:
+    a = np.frombuffer(raw[:32], dtype=np.uint8).copy()
+    h = VdifHeader()
+    call("vfrb_vdif_decode", _ptr(a, C.c_uint8), C.byref(h))
+    return {k: getattr(h, k) for k, _ in h._fields_}
+
+
+def vdif_encode(**fields) -> bytes:
+    h = VdifHeader()
+    for k, v in fields.items():
+        setattr(h, k, v)
+    out = np.zeros(32, dtype=np.uint8)
+    call("vfrb_vdif_encode", C.byref(h), _ptr(out, C.c_uint8))
+    return out.tobytes()
+
+
+def vdif_dmjd(raw: bytes, fps=25600) -> float:
+    a = np.frombuffer(raw[:32], dtype=np.uint8).copy()
+    return lib.vfrb_vdif_dmjd(_ptr(a, C.c_uint8), fps)
+
+
+def sigproc_header(fch1, foff, nchans, nbits, tstart, tsamp) -> bytes:
+    buf = np.zeros(1024, dtype=np.uint8)
+    n = i32()
+    call("vfrb_sigproc_header", _ptr(buf, C.c_uint8), 1024, C.byref(n), fch1, foff, nchans, nbits,
+         tstart, tsamp)
+    return buf[: n.value].tobytes()
+
+
+def presto_inf(base, mjdstart, ntimes, tsamp, dm, fchanlo, bw, nchan, chanbw) -> str:
+    buf = C.create_string_buffer(4096)
+    call("vfrb_presto_inf", buf, 4096, base.encode(), mjdstart, ntimes, tsamp, dm, fchanlo, bw,
+         nchan, chanbw)
+    return buf.value.decode()
+
+
+class Sequencer:
+    """Frame sequencer over an in-memory VDIF byte string."""
+
+    def __init__(self, data: bytes | None = None, path: str | None = None):
+        self.h = vp()
+        self.log_lines: list[str] = []
+        if path is not None:
+            call("vfrb_seq_open_file", C.byref(self.h), path.encode())
+        else:
+            self._buf = np.frombuffer(data, dtype=np.uint8).copy()
+            call("vfrb_seq_open_memory", C.byref(self.h), _ptr(self._buf, C.c_uint8), self._buf.size)
+        self._cb = LOG_FN(lambda line, user: self.log_lines.append(line.decode()))
+        call("vfrb_seq_set_log", self.h, self._cb, None)
+
+    def init(self) -> int:
+        return lib.vfrb_seq_init(self.h)
+
+    def fill(self, nloop):
+        a = np.empty(nloop, dtype=np.uint8)
+        b = np.empty(nloop, dtype=np.uint8)
+        st = lib.vfrb_seq_fill(self.h, nloop, _ptr(a, C.c_uint8), _ptr(b, C.c_uint8))
+        return st, a, b
+
+    def header(self) -> bytes:
+        a = np.zeros(32, dtype=np.uint8)
+        call("vfrb_seq_header", self.h, _ptr(a, C.c_uint8))
+        return a.tobytes()
+
+    def counters(self):
+        r, f = i64(), i64()
+        call("vfrb_seq_counters", self.h, C.byref(r), C.byref(f))
+        return r.value, f.value
+
+    def close(self):
+        if self.h:
+            lib.vfrb_seq_close(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        self.close()
+
+
+def synth_spec(nfft, seed=0, pulses=(), tones=(), mean=128.0, sigma=8.0, nthreads=0) -> SynthSpec:
+    """pulses: (dm, t0, fwhm, amp); tones: (bin, amp)."""
+    s = SynthSpec()
+    s.seed, s.nfft, s.mean, s.sigma = seed, nfft, mean, sigma
+    s.f_top_mhz, s.bw_mhz, s.nthreads = 384.0, 64.0, nthreads
+    s.npulse = len(pulses)
+    for i, (dm, t0, w, a) in enumerate(pulses):
+        s.pulse_dm[i], s.pulse_t0[i], s.pulse_fwhm[i], s.pulse_amp[i] = dm, t0, w, a
+    s.ntone = len(tones)
+    for i, (b, a) in enumerate(tones):
+        s.tone_bin[i], s.tone_amp[i] = b, a
+    return s
+
+
+def synth_payload(spec: SynthSpec, pol: int, block0: int, nblocks: int, out=None) -> np.ndarray:
+    if out is None:
+        out = np.empty(nblocks * spec.nfft, dtype=np.uint8)
+    call("vfrb_synth_payload", C.byref(spec), pol, block0, nblocks, _ptr(out, C.c_uint8))
+    return out
+
+
+def amp_for_snr(snr, nave, nchan) -> float:
+    return lib.vfrb_synth_amp_for_snr(snr, nave, nchan)
+
+
+# --------------------------------------------------------------------------
+# Device helpers
+# --------------------------------------------------------------------------
+def device_count() -> int:
+    n = i32()
+    call("vfrb_device_count", C.byref(n))
+    return n.value
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    call("vfrb_device_name", buf, 256)
+    return buf.value.decode()
+
+
+def mem_info():
+    f, t = sz(), sz()
+    call("vfrb_mem_info", C.byref(f), C.byref(t))
+    return f.value, t.value
+
+
+class DeviceArray:
+    """A device allocation with a numpy shape/dtype for host round trips."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = vp()
+        call("vfrb_malloc", C.byref(self.ptr), max(self.nbytes, 1))
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a, stream)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        call("vfrb_memcpy_h2d", self.ptr, a.ctypes.data_as(vp), a.nbytes, stream)
+        if stream:
+            call("vfrb_stream_synchronize", stream)
+
+    def download(self, stream=None) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        call("vfrb_memcpy_d2h", out.ctypes.data_as(vp), self.ptr, self.nbytes, stream)
+        if stream:
+            call("vfrb_stream_synchronize", stream)
+        return out
+
+    def zero(self, stream=None):
+        call("vfrb_memset", self.ptr, 0, self.nbytes, stream)
+
+    def offset(self, nbytes: int) -> vp:
+        return vp(self.ptr.value + int(nbytes))
+
+    def free(self):
+        if self.ptr:
+            lib.vfrb_free(self.ptr)
+            self.ptr = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        self.h = vp()
+        call("vfrb_stream_create", C.byref(self.h))
+
+    def sync(self):
+        call("vfrb_stream_synchronize", self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_stream_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = vp()
+        call("vfrb_event_create", C.byref(self.h))
+
+    def record(self, stream=None):
+        call("vfrb_event_record", self.h, stream.h if isinstance(stream, Stream) else stream)
+
+    def sync(self):
+        call("vfrb_event_synchronize", self.h)
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = f32()
+        call("vfrb_event_elapsed_ms", C.byref(ms), self.h, later.h)
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def sync():
+    call("vfrb_device_synchronize")
+
+
+def _s(stream):
+    return stream.h if isinstance(stream, Stream) else stream
+
+
+class FilterbankPlan:
+    def __init__(self, twonchan, nave, nskip, nchanout):
+        self.h = vp()
+        self.twonchan, self.nave, self.nskip, self.nchanout = twonchan, nave, nskip, nchanout
+        call("vfrb_fb_plan_create", C.byref(self.h), twonchan, nave, nskip, nchanout)
+
+    def execute(self, pol0: DeviceArray, pol1: DeviceArray, nrows, fb: DeviceArray, ring_rows=0,
+                row0=0, stream=None, pol_offset=0):
+        call("vfrb_fb_execute", self.h, pol0.offset(pol_offset), pol1.offset(pol_offset), nrows,
+             fb.ptr, ring_rows, row0, _s(stream))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_fb_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+class SearchPlan:
+    def __init__(self, nchans, tsamp, fch1, chanbw, dms, max_rows, storage=STORE_F32, tile=-1):
+        self._dms = np.ascontiguousarray(dms, dtype=np.float32)
+        d = SearchDesc(nchans, tsamp, fch1, chanbw, len(self._dms), _ptr(self._dms, f32), max_rows,
+                       storage, tile)
+        self.h = vp()
+        self.nchans, self.ndm, self.storage = nchans, len(self._dms), storage
+        call("vfrb_search_plan_create", C.byref(self.h), C.byref(d))
+        self.max_delay = lib.vfrb_search_max_delay(self.h)
+
+    def offsets(self) -> np.ndarray:
+        off = np.empty((self.ndm, self.nchans), dtype=np.int32)
+        call("vfrb_search_get_offsets", self.h, _ptr(off, i32))
+        return off
+
+    def load(self, fb: DeviceArray, nrows, ring_rows=0, row0=0, bandpass=True, stream=None):
+        call("vfrb_search_load", self.h, fb.ptr, ring_rows, row0, nrows, int(bool(bandpass)), _s(stream))
+
+    def dedisperse(self, nrows, out: DeviceArray, ld_out, stream=None):
+        call("vfrb_search_dedisperse", self.h, nrows, out.ptr, ld_out, _s(stream))
+
+    def execute(self, nrows, fb_tm: DeviceArray, out: DeviceArray, ld_out, stream=None):
+        call("vfrb_search_execute", self.h, nrows, fb_tm.ptr, out.ptr, ld_out, _s(stream))
+
+    def dedisperse_stats(self, nrows, out: DeviceArray, sig: DeviceArray, ld_out, stream=None):
+        call("vfrb_search_dedisperse_stats", self.h, nrows, out.ptr, sig.ptr, ld_out, _s(stream))
+
+    def set_tile(self, i):
+        call("vfrb_search_set_tile", self.h, i)
+
+    @property
+    def tile(self):
+        return lib.vfrb_search_get_tile(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_search_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+def tile_names():
+    return [lib.vfrb_search_tile_name(i).decode() for i in range(lib.vfrb_search_tile_count())]
+
+
+def bandpass_inplace(fb: DeviceArray, nrows, nch, stream=None):
+    call("vfrb_bandpass_inplace", fb.ptr, nrows, nch, _s(stream))
+
+
+def chan_stats(fb: DeviceArray, nrows, nch, ncol, nblk, stream=None):
+    ave, sig, out = (DeviceArray((nblk, nch), np.float32) for _ in range(3))
+    call("vfrb_chan_stats", fb.ptr, nrows, nch, ncol, nblk, ave.ptr, sig.ptr, out.ptr, _s(stream))
+    return ave.download(), sig.download(), out.download()
+
+
+@dataclass
+class DmStats:
+    frb: np.ndarray
+    ave: np.ndarray
+    sigma: np.ndarray
+    peak: np.ndarray
+    peak_t: np.ndarray
+
+
+def dm_stats(out: DeviceArray, ld, ndm, nt, acc=ACC_F64, stream=None) -> DmStats:
+    frb = DeviceArray(ndm, np.int32)
+    ave, sigma, peak = (DeviceArray(ndm, np.float32) for _ in range(3))
+    pt = DeviceArray(ndm, np.int32)
+    call("vfrb_dm_stats", out.ptr, ld, ndm, nt, acc, frb.ptr, ave.ptr, sigma.ptr, peak.ptr, pt.ptr,
+         _s(stream))
+    return DmStats(frb.download(), ave.download(), sigma.download(), peak.download(), pt.download())
+
+
+def candidates(out: DeviceArray, ld, ndm, nt, ave: np.ndarray, sigma: np.ndarray, threshold,
+               max_cand=1 << 16, stream=None):
+    d_ave = DeviceArray.from_host(np.asarray(ave, np.float32))
+    d_sig = DeviceArray.from_host(np.asarray(sigma, np.float32))
+    cdm, ct = DeviceArray(max_cand, np.int32), DeviceArray(max_cand, np.int32)
+    cs = DeviceArray(max_cand, np.float32)
+    cnt = DeviceArray(1, np.int32)
+    call("vfrb_candidates", out.ptr, ld, ndm, nt, d_ave.ptr, d_sig.ptr, float(threshold), cdm.ptr,
+         ct.ptr, cs.ptr, max_cand, cnt.ptr, _s(stream))
+    n = int(cnt.download()[0])
+    k = min(n, max_cand)
+    return n, cdm.download()[:k], ct.download()[:k], cs.download()[:k]
