@@ -327,6 +327,13 @@ typedef void (*vfrb_log_fn)(const char* line, void* user);
 vfrb_status vfrb_seq_open_file(vfrb_seq** seq, const char* path);
 vfrb_status vfrb_seq_open_memory(vfrb_seq** seq, const uint8_t* data, size_t bytes);
 vfrb_status vfrb_seq_set_log(vfrb_seq* seq, vfrb_log_fn fn, void* user);
+/* Gap rule.  0 (default): frame-number only, gap = (fnum - expected) mod
+ * 25600 (PIPELINE:888-930).  1: seconds-aware, gap = fnum - expected +
+ * (sec - expected_sec)*25600 and the pols must also agree on seconds at
+ * init (TIMESERIES:893-937,1095-1140). */
+vfrb_status vfrb_seq_set_seconds_check(vfrb_seq* seq, int32_t enable);
+/* initDADA (TIMESERIES:767-826): discard npairs raw frame pairs before init. */
+vfrb_status vfrb_seq_skip_pairs(vfrb_seq* seq, int64_t npairs);
 /* initframe01 + initskip01 (PIPELINE:1044-1131).  Returns VFRB_ERR_EOF at
  * end of input and VFRB_ERR_FORMAT on an alignment failure. */
 vfrb_status vfrb_seq_init(vfrb_seq* seq);

@@ -44,7 +44,12 @@ def build_stream(frames) -> bytes:
 
 
 class RefSequencer:
-    def __init__(self, data: bytes):
+    """seconds=True selects the seconds-aware gap rule and init checks of
+    TIMESERIES (vdifDADA2timeseries.cu:893-937,1095-1140)."""
+
+    def __init__(self, data: bytes, seconds: bool = False):
+        self.seconds = seconds
+        self.expected_sec = [0, 0]
         self.data = data
         self.pos = 0
         self.hdr = [b"", b""]
@@ -82,8 +87,20 @@ class RefSequencer:
     def tid(self, pol):
         return parse_header(self.hdr[pol])["thread"]
 
+    def fsec(self, pol):
+        return parse_header(self.hdr[pol])["seconds"]
+
     def checkframeskip(self, pol):
         f = self.fnum(pol)
+        if self.seconds:
+            s = self.fsec(pol)
+            if f != self.expected[pol] or s != self.expected_sec[pol]:
+                self.nskip[pol] = f - self.expected[pol] + (s - self.expected_sec[pol]) * FPS
+                self.tmp[pol] = bytearray(self.buf[pol])
+                self.buf[pol] = bytearray(self.skip[pol])
+            self.expected[pol] = (f + 1) % FPS
+            self.expected_sec[pol] = s + (1 if self.expected[pol] == 0 else 0)
+            return
         if f != self.expected[pol]:
             self.nskip[pol] = f - self.expected[pol]
             if self.nskip[pol] < 0:
@@ -123,7 +140,7 @@ class RefSequencer:
             return 1
         if self.tid(1) != 1:
             return 2
-        if self.fnum(0) != self.fnum(1):
+        if self.fnum(0) != self.fnum(1) or (self.seconds and self.fsec(0) != self.fsec(1)):
             return 2
         f0 = self.fnum(0)
         while f0 % 2560 != 0:
@@ -132,9 +149,11 @@ class RefSequencer:
             if self.initframe(1):
                 return 1
             f0 = self.fnum(0)
-        if f0 != self.fnum(1):
+        if f0 != self.fnum(1) or (self.seconds and self.fsec(0) != self.fsec(1)):
             return 2
         self.expected = [(f0 + 1) % FPS, (self.fnum(1) + 1) % FPS]
+        if self.seconds:
+            self.expected_sec = [self.fsec(p) + (1 if self.expected[p] == 0 else 0) for p in (0, 1)]
         return 0
 
     def fill(self, nloop):
@@ -160,9 +179,9 @@ class RefSequencer:
         return 0, bytes(t0), bytes(t1)
 
 
-def reference_bundles(data: bytes, nloop: int, max_bundles=10**9):
+def reference_bundles(data: bytes, nloop: int, max_bundles=10**9, seconds=False):
     """All bundles the reference main loop would process, and the init status."""
-    s = RefSequencer(data)
+    s = RefSequencer(data, seconds)
     st = s.initframe01()
     if st:
         return st, []

@@ -179,7 +179,12 @@ def test_load_from_ring(gpu, orc):
     assert np.max(np.abs(out.download() - ref)) < 2e-3
 
 
-@pytest.mark.parametrize("storage,tol", [("f16", 2e-3), ("u8", 2.5e-2)])
+# Tolerances, as max |err| / std(dedispersed series) over 171k samples:
+#  f16: rounding 2^-11 relative per element, averaged over 1024 channels.
+#  u8 : per-channel step = range/255 ~ 9 sigma_c/255, uniform rounding error
+#       sigma_c*9/(255*sqrt(12)) = 1.0% of the series std after the channel
+#       mean; the maximum over 1.7e5 Gaussian samples is ~4.6 of those std.
+@pytest.mark.parametrize("storage,tol", [("f16", 2e-3), ("u8", 6e-2)])
 def test_reduced_precision_storage(gpu, orc, storage, tol):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()

@@ -155,6 +155,44 @@ def test_sequencer_matches_reference_restatement(vfrb, case, nloop):
         assert any("FRAME SKIP" in line for line in seq.log_lines)
 
 
+@pytest.mark.parametrize("case", [dict(), dict(drop=(5,)), dict(start_frame=25590, drop=(12, 13)),
+                                  dict(start_frame=2555)])
+def test_sequencer_seconds_aware_matches_restatement(vfrb, case):
+    data = _stream(40, **case)
+    nloop = 4096 * 3
+    st, ref = vo.reference_bundles(data, nloop, seconds=True)
+    seq = vfrb.Sequencer(data, seconds_check=True)
+    assert st == 0 and seq.init() == 0
+    got = []
+    while True:
+        s, a, b = seq.fill(nloop)
+        if s != 0:
+            break
+        got.append((a.tobytes(), b.tobytes()))
+    assert [(a, b) for a, b in got] == ref
+
+
+def test_sequencer_seconds_aware_sees_a_whole_second_gap(vfrb):
+    # 25600 frames missing: same frame number, next second.  The frame-number
+    # rule sees no gap, the seconds-aware rule inserts a second of zeros.
+    rng = np.random.default_rng(1)
+    fr = []
+    for i in range(6):
+        sec = 100 + (1 if i >= 3 else 0)
+        for th in (0, 1):
+            fr.append((sec, i, th, rng.integers(1, 256, 5000, dtype=np.uint8)))
+    data = vo.build_stream(fr)
+    plain = vfrb.Sequencer(data)
+    aware = vfrb.Sequencer(data, seconds_check=True)
+    assert plain.init() == 0 and aware.init() == 0
+    st, a, _ = plain.fill(5000 * 4)
+    assert st == 0 and np.all(a != 0)
+    st, a, _ = aware.fill(5000 * 4)
+    assert st == 0 and np.all(a[3 * 5000:] == 0)
+    _, ref = vo.reference_bundles(data, 5000 * 4, 1, seconds=True)
+    assert a.tobytes() == ref[0][0]
+
+
 def test_sequencer_gap_is_zero_filled(vfrb):
     data = _stream(12, drop=(4, 5))
     seq = vfrb.Sequencer(data)

@@ -116,7 +116,9 @@ struct vfrb_seq {
     uint8_t data[2][VFRB_DATALEN];
     uint8_t held[2][VFRB_DATALEN];  // real frame parked while zeros are emitted
     int32_t expected[2] = {0, 0};
+    int32_t expected_sec[2] = {0, 0};
     int32_t pending_zero[2] = {0, 0};
+    bool sec_check = false;
     // bundle carry
     int32_t part_fill = 0, part_left = 0;
     int64_t frames_read = 0, frames_filled = 0;
@@ -165,6 +167,11 @@ struct vfrb_seq {
         vfrb_vdif_decode(hdr[pol], &h);
         return (int)h.frame;
     }
+    int full_second(int pol) const {
+        vfrb_vdif_header h;
+        vfrb_vdif_decode(hdr[pol], &h);
+        return (int)h.seconds;
+    }
     int thread_id(int pol) const {
         vfrb_vdif_header h;
         vfrb_vdif_decode(hdr[pol], &h);
@@ -174,19 +181,31 @@ struct vfrb_seq {
     // checkframeskip0/1 (PIPELINE:888-930): a frame number other than the
     // expected one starts a run of zero-filled payloads whose length is the
     // gap (mod FRAMESPERSEC); the real frame is parked until the run ends.
+    //
+    // Seconds-aware rule of TIMESERIES:893-937: the gap also counts whole
+    // seconds, gap = fnum - expected + (sec - expected_sec) * FRAMESPERSEC.
     void check_gap(int pol) {
         const int fnum = frame_number(pol);
-        if (fnum != expected[pol]) {
+        const int fsec = sec_check ? full_second(pol) : 0;
+        if (fnum != expected[pol] || (sec_check && fsec != expected_sec[pol])) {
             int gap = fnum - expected[pol];
-            if (gap < 0) gap += VFRB_FRAMESPERSEC;
+            if (sec_check) {
+                gap += (fsec - expected_sec[pol]) * VFRB_FRAMESPERSEC;
+                if (gap < 0) gap = 0;  // frames going backwards: no zero run
+            } else if (gap < 0) {
+                gap += VFRB_FRAMESPERSEC;
+            }
             pending_zero[pol] = gap;
             log("FRAME SKIP %d!!! EXPECTED FRAME NUMBER%d= %d, BUT READ %d! (NSKIP%d= %d)\n", pol, pol,
                 expected[pol], fnum, pol, gap);
-            std::memcpy(held[pol], data[pol], VFRB_DATALEN);
-            std::memset(data[pol], 0, VFRB_DATALEN);
-            frames_filled++;
+            if (gap > 0) {
+                std::memcpy(held[pol], data[pol], VFRB_DATALEN);
+                std::memset(data[pol], 0, VFRB_DATALEN);
+                frames_filled++;
+            }
         }
         expected[pol] = (fnum + 1) % VFRB_FRAMESPERSEC;
+        expected_sec[pol] = fsec + (sec_check && expected[pol] == 0 ? 1 : 0);
     }
 
     // readoneframePOL0/1 (PIPELINE:934-1025).
@@ -234,6 +253,20 @@ vfrb_status vfrb_seq_open_memory(vfrb_seq** seq, const uint8_t* data, size_t byt
     return VFRB_OK;
 }
 
+vfrb_status vfrb_seq_skip_pairs(vfrb_seq* seq, int64_t npairs) {
+    VFRB_REQUIRE(seq && npairs >= 0, VFRB_ERR_INVALID, "bad skip arguments");
+    for (int64_t i = 0; i < npairs; i++) {
+        if (seq->read_raw(0) || seq->read_raw(1)) return VFRB_ERR_EOF;
+    }
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_seq_set_seconds_check(vfrb_seq* seq, int32_t enable) {
+    VFRB_REQUIRE(seq, VFRB_ERR_INVALID, "NULL sequencer");
+    seq->sec_check = enable != 0;
+    return VFRB_OK;
+}
+
 vfrb_status vfrb_seq_set_log(vfrb_seq* seq, vfrb_log_fn fn, void* user) {
     VFRB_REQUIRE(seq, VFRB_ERR_INVALID, "NULL sequencer");
     seq->log_fn = fn;
@@ -265,7 +298,7 @@ vfrb_status vfrb_seq_init(vfrb_seq* s) {
         return VFRB_ERR_FORMAT;
     }
     int fnum0 = s->frame_number(0), fnum1 = s->frame_number(1);
-    if (fnum0 != fnum1) {
+    if (fnum0 != fnum1 || (s->sec_check && s->full_second(0) != s->full_second(1))) {
         s->log("ERROR! polarization VDIF frame numbers do not agree! 0: %d  !=  1: %d\n", fnum0, fnum1);
         vfrb::set_error("pol frame numbers differ: %d != %d", fnum0, fnum1);
         return VFRB_ERR_FORMAT;
@@ -282,13 +315,15 @@ vfrb_status vfrb_seq_init(vfrb_seq* s) {
     }
     s->log("Skipped %d frames to begin at noise cal cycle\n", ncal);
     fnum1 = s->frame_number(1);
-    if (fnum0 != fnum1) {
+    if (fnum0 != fnum1 || (s->sec_check && s->full_second(0) != s->full_second(1))) {
         s->log("ERROR! polarization VDIF frame numbers do not agree! 0: %d  !=  1: %d\n", fnum0, fnum1);
         vfrb::set_error("pol frame numbers differ after cal alignment: %d != %d", fnum0, fnum1);
         return VFRB_ERR_FORMAT;
     }
     s->expected[0] = (fnum0 + 1) % VFRB_FRAMESPERSEC;
     s->expected[1] = (fnum1 + 1) % VFRB_FRAMESPERSEC;
+    for (int pol = 0; pol < 2; pol++)
+        s->expected_sec[pol] = s->full_second(pol) + (s->expected[pol] == 0 ? 1 : 0);
     s->initialised = true;
     return VFRB_OK;
 }

@@ -124,6 +124,8 @@ _SIGS = {
     "vfrb_seq_open_file": (i32, [P(vp), C.c_char_p]),
     "vfrb_seq_open_memory": (i32, [P(vp), P(C.c_uint8), sz]),
     "vfrb_seq_set_log": (i32, [vp, LOG_FN, vp]),
+    "vfrb_seq_set_seconds_check": (i32, [vp, i32]),
+    "vfrb_seq_skip_pairs": (i32, [vp, i64]),
     "vfrb_seq_init": (i32, [vp]),
     "vfrb_seq_header": (i32, [vp, P(C.c_uint8)]),
     "vfrb_seq_fill": (i32, [vp, i32, P(C.c_uint8), P(C.c_uint8)]),
@@ -215,7 +217,7 @@ def presto_inf(base, mjdstart, ntimes, tsamp, dm, fchanlo, bw, nchan, chanbw) ->
 class Sequencer:
     """Frame sequencer over an in-memory VDIF byte string."""
 
-    def __init__(self, data: bytes | None = None, path: str | None = None):
+    def __init__(self, data: bytes | None = None, path: str | None = None, seconds_check=False):
         self.h = vp()
         self.log_lines: list[str] = []
         if path is not None:
@@ -225,6 +227,7 @@ class Sequencer:
             call("vfrb_seq_open_memory", C.byref(self.h), _ptr(self._buf, C.c_uint8), self._buf.size)
         self._cb = LOG_FN(lambda line, user: self.log_lines.append(line.decode()))
         call("vfrb_seq_set_log", self.h, self._cb, None)
+        call("vfrb_seq_set_seconds_check", self.h, int(bool(seconds_check)))
 
     def init(self) -> int:
         return lib.vfrb_seq_init(self.h)
