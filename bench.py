@@ -92,7 +92,15 @@ def main():
         snr = rng.uniform(*W["snr"])
         t0 = rng.uniform(0.1, W["seg_s"] * 0.6)
         pulses.append((dm, t0, w, vfrb.amp_for_snr(snr, nave, nch)))
-    tones = [(float(rng.integers(1, n // 2)), 40.0) for _ in range(W["ntone"])]
+    # Narrowband tones "at 5 sigma": the tone's channel power sits 5 filterbank
+    # noise sigmas above the channel mean.  Per FFT frame and pol a tone of
+    # amplitude a puts (a*N/2)^2 in its bin against N*sigma^2 of noise, and
+    # the averaged noise power has relative sigma 1/sqrt(2*nave), so
+    # a = 2*sigma*sqrt(5/(N*sqrt(2*nave))).  (5 sigma of the voltage, 40
+    # counts, would put ~5e4 x the noise into the channel and swamp the
+    # channel sum of the mean-subtracted filterbank.)
+    tone_amp = 2.0 * 8.0 * np.sqrt(5.0 / (n * np.sqrt(2.0 * nave)))
+    tones = [(float(rng.integers(1, n // 2)), tone_amp) for _ in range(W["ntone"])]
     spec = vfrb.synth_spec(n, seed=W["seed"], pulses=pulses, tones=tones)
     nblk = seg_rows * nave
     t = time.perf_counter()
@@ -129,6 +137,14 @@ def main():
     for sname in storages:
         sp = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, win_rows, vfrb.STORAGE_NAMES[sname], args.tile)
         acc = vfrb.ACC_F64 if sname == "f32" else vfrb.ACC_F32C
+        tune = None
+        if args.tile < 0:
+            # autotune the dedispersion tile on a real window (not timed)
+            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=max_delay, stream=s)
+            sp.load(ring, win_rows, ring_rows=ring_rows, row0=0, bandpass=True, stream=s)
+            tune = sp.autotune(win_rows, out, seg_rows, iters=2, stream=s)
+            log(f"{sname}: autotuned tile {vfrb.tile_names()[sp.tile]}: " +
+                " ".join(f"{k}={v:.2f}" for k, v in tune.items()))
         ev = [vfrb.Event() for _ in range(6)]
         names = ["filterbank", "bandpass_pack", "dedisperse", "dm_stats", "candidates"]
         stage_ms = {k: [] for k in names}
@@ -176,7 +192,7 @@ def main():
                               stage_ms={k: float(np.mean(v)) for k, v in stage_ms.items()},
                               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                               candidates=ncand, best_dm=float(dms[best]), best_snr=float(pk[best]),
-                              tile=vfrb.tile_names()[sp.tile])
+                              tile=vfrb.tile_names()[sp.tile], autotune_ms=tune)
         log(f"{sname}: RTF={rtf:.1f} ms/step={ms_step:.2f} stages="
             + " ".join(f"{k}={np.mean(v):.3f}" for k, v in stage_ms.items())
             + f" cand={ncand} best DM={dms[best]:.1f} SNR={pk[best]:.1f}")
@@ -251,6 +267,8 @@ def main():
         "p99_ms_by_storage": {k: v["p99_ms"] for k, v in results.items()},
         "stage_ms_by_storage": {k: v["stage_ms"] for k, v in results.items()},
         "candidates_by_storage": {k: v["candidates"] for k, v in results.items()},
+        "tile_by_storage": {k: v["tile"] for k, v in results.items()},
+        "autotune_ms_by_storage": {k: v["autotune_ms"] for k, v in results.items()},
         "stages": stages_roof,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -267,6 +267,16 @@ int32_t     vfrb_search_tile_count(void);
 const char* vfrb_search_tile_name(int32_t index);
 vfrb_status vfrb_search_set_tile(vfrb_search_plan* plan, int32_t index);
 int32_t     vfrb_search_get_tile(const vfrb_search_plan* plan);
+/*
+ * Autotuner (no reference counterpart): times every tile configuration
+ * (samples per lane x DMs per wave x waves per workgroup) that fits the plan's
+ * DM list on the window currently loaded, `iters` launches each, and keeps
+ * the fastest.  ms (optional, host, vfrb_search_tile_count() floats) gets the
+ * average milliseconds per launch, -1 where a configuration does not fit.
+ * out is scratch with the vfrb_search_dedisperse shape.
+ */
+vfrb_status vfrb_search_autotune(vfrb_search_plan* plan, int32_t nrows, float* out,
+                                 int64_t ld_out, int32_t iters, float* ms, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Per-DM statistics and candidates                                         */

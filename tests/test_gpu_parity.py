@@ -125,7 +125,7 @@ def test_delay_table_device_matches_oracle(gpu, orc):
     assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
 
 
-@pytest.mark.parametrize("tile", range(8))
+@pytest.mark.parametrize("tile", range(12))
 def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()
@@ -142,6 +142,27 @@ def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     ref = orc.dedisperse_f32(fb, plan.offsets(), nt)
     # same terms, different summation tree: f32 rounding only
     assert np.max(np.abs(got - ref)) < 1e-4 * 22
+
+
+def test_autotune_picks_a_fitting_tile_and_keeps_results(gpu, orc):
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(21)
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    for st in ("f32", "f16"):
+        plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[st])
+        nt = rows - plan.max_delay
+        d = vfrb.DeviceArray.from_host(fb)
+        out = vfrb.DeviceArray((len(dms), nt), np.float32)
+        plan.load(d, rows, bandpass=False)
+        plan.dedisperse(rows, out, nt)
+        before = out.download()
+        ms = plan.autotune(rows, out, nt, iters=1)
+        assert plan.tile >= 0 and ms[vfrb.tile_names()[plan.tile]] == min(v for v in ms.values() if v > 0)
+        plan.dedisperse(rows, out, nt)
+        after = out.download()
+        assert np.max(np.abs(after - before)) < 1e-4 * 22
 
 
 def test_load_bandpass_then_dedisperse_matches_oracle_chain(gpu, orc):

@@ -196,22 +196,31 @@ __global__ void range_init_kernel(unsigned* range, int nch) {
 // ---------------------------------------------------------------------------
 // Dedispersion
 // ---------------------------------------------------------------------------
-template <typename TS>
-__device__ __forceinline__ float to_f32(TS v);
-template <>
-__device__ __forceinline__ float to_f32<float>(float v) { return v; }
-template <>
-__device__ __forceinline__ float to_f32<__half>(__half v) { return __half2float(v); }
-template <>
-__device__ __forceinline__ float to_f32<uint8_t>(uint8_t v) { return (float)v; }
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+constexpr h2_t kSelLo = {(_Float16)1.0f, (_Float16)0.0f};
+constexpr h2_t kSelHi = {(_Float16)0.0f, (_Float16)1.0f};
 
 struct Chunk {
-    int c0, count, stride, pad;
+    int c0, count, stride, pad;  // stride: staged samples per copy, multiple of 4
+};
+
+// At most kChunkMax channels per LDS chunk; staged rows take at most
+// kDataBudget bytes, so with the offset table a workgroup stays under 80 KB
+// and two fit in one CU's 160 KB of LDS.
+constexpr int kChunkMax = 32;
+constexpr size_t kDataBudget = 60 * 1024;
+
+// LDS (<= 76 KB per workgroup) limits a CU to two workgroups, i.e. W/2 waves
+// per SIMD: let the register allocator use the VGPR budget that occupancy
+// implies (512 / (W/2) per lane), so that many LDS reads can be in flight.
+template <int W>
+struct WavesPerEu {
+    static constexpr int value = W / 2 > 1 ? W / 2 : 1;
 };
 
 // SL samples per lane, DW DMs per wave, W waves; T = 64*SL samples per tile.
 template <typename TS, int SL, int DW, int W>
-__global__ void __launch_bounds__(64 * W)
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
               const uint16_t* __restrict__ rel,  // [nblk][nch][W*DW]
               const int* __restrict__ dmin,      // [nblk][nch]
@@ -219,10 +228,19 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
               const float* __restrict__ chan_scale,  // u8 only
               const int* __restrict__ f16_exp, const float* __restrict__ offsum,
               float* __restrict__ out, int64_t ld_out) {
+    // LDS: [kChunkMax][DMB] u32 byte offsets of each (channel, DM) read,
+    // then the staged rows.  A row holds COPIES copies of the channel's
+    // samples, copy p shifted by p elements, so that the dword a lane reads
+    // (EPD consecutive samples) is always 4-byte aligned whatever the delay.
     extern __shared__ unsigned char lds_raw[];
-    TS* lds = reinterpret_cast<TS*>(lds_raw);
+    constexpr int ESZ = (int)sizeof(TS);
+    constexpr int EPD = 4 / ESZ;           // samples per dword = copies per row
     constexpr int T = 64 * SL;
     constexpr int DMB = W * DW;
+    constexpr int NDW = SL / EPD;          // dwords per lane per (DM, channel)
+    static_assert(SL % EPD == 0, "SL must hold whole dwords");
+    uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
+    unsigned char* data = lds_raw + kChunkMax * DMB * 4;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int b = blockIdx.y;
@@ -238,31 +256,54 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
 
     for (int k = 0; k < cr.y; k++) {
         const Chunk ch = chunks[cr.x + k];
-        // stage window[c][t0 + dmin_c + s], s < stride, rows >= nrows -> 0
+        const int rowb = 4 * ch.stride;    // bytes per staged channel (all copies)
+        // stage copy p: element i = window[c][t0 + dmin_c + i + p], rows >= nrows -> 0
         for (int ci = wave; ci < ch.count; ci += W) {
             const int c = ch.c0 + ci;
             const int64_t src0 = (int64_t)c * ld;
             const int r0 = t0 + dmin[(int64_t)b * nch + c];
-            TS* dst = lds + ci * ch.stride;
-            for (int s = lane; s < ch.stride; s += 64) {
+            unsigned char* row = data + ci * rowb;
+            for (int s = lane; s < ch.stride + EPD - 1; s += 64) {
                 const int r = r0 + s;
-                dst[s] = r < nrows ? win[src0 + r] : TS(0);
+                const TS v = r < nrows ? win[src0 + r] : TS(0);
+#pragma unroll
+                for (int p = 0; p < EPD; p++) {
+                    const int i = s - p;
+                    if (i >= 0 && i < ch.stride)
+                        reinterpret_cast<TS*>(row + p * ch.stride * ESZ)[i] = v;
+                }
             }
+        }
+        // byte offset of the first dword of each (channel, DM) read
+        for (int idx = threadIdx.x; idx < ch.count * DMB; idx += 64 * W) {
+            const int ci = idx / DMB, d = idx - ci * DMB;
+            const int r = rel[((int64_t)b * nch + ch.c0 + ci) * DMB + d];
+            const int p = r & (EPD - 1);
+            roff[ci * DMB + d] = (uint32_t)(ci * rowb + p * ch.stride * ESZ + (r - p) * ESZ);
         }
         __syncthreads();
         for (int ci = 0; ci < ch.count; ci++) {
-            const int c = ch.c0 + ci;
-            const uint16_t* rp = rel + ((int64_t)b * nch + c) * DMB + wave * DW;
-            const TS* row = lds + ci * ch.stride + lane;
+            const uint32_t* ro = roff + ci * DMB + wave * DW;
             float sc = 1.0f;
-            if constexpr (sizeof(TS) == 1) sc = chan_scale[c];
+            if constexpr (ESZ == 1) sc = chan_scale[ch.c0 + ci];
 #pragma unroll
             for (int d = 0; d < DW; d++) {
-                const TS* p = row + rp[d];
+                const uint32_t* p = reinterpret_cast<const uint32_t*>(data + ro[d] + 4 * lane);
 #pragma unroll
-                for (int j = 0; j < SL; j++) {
-                    if constexpr (sizeof(TS) == 1) acc[d][j] = fmaf(sc, to_f32(p[64 * j]), acc[d][j]);
-                    else acc[d][j] += to_f32(p[64 * j]);
+                for (int q = 0; q < NDW; q++) {
+                    const uint32_t w = p[64 * q];
+                    if constexpr (ESZ == 4) {
+                        acc[d][q] += __uint_as_float(w);
+                    } else if constexpr (ESZ == 2) {
+                        // one v_dot2c_f32_f16 per sample: acc += h.lo*1 + h.hi*0
+                        const h2_t h = *reinterpret_cast<const h2_t*>(&w);
+                        acc[d][2 * q] = __builtin_amdgcn_fdot2(h, kSelLo, acc[d][2 * q], false);
+                        acc[d][2 * q + 1] = __builtin_amdgcn_fdot2(h, kSelHi, acc[d][2 * q + 1], false);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            acc[d][4 * q + e] = fmaf(sc, (float)((w >> (8 * e)) & 0xffu), acc[d][4 * q + e]);
+                    }
                 }
             }
         }
@@ -270,22 +311,27 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     }
     // f32: sum / nch, the reference's final division (PIPELINE:1535);
     // f16: undo the power-of-two storage scale; u8: add the channel offsets.
+    // Lane element (q, e) is sample t0 + EPD*lane + 64*EPD*q + e.
     const float fnch = (float)nch;
     float mul = 1.0f, add = 0.0f;
-    if constexpr (sizeof(TS) == 2) mul = ldexpf(1.0f, f16_exp[0]);
-    if constexpr (sizeof(TS) == 1) add = offsum[0];
+    if constexpr (ESZ == 2) mul = ldexpf(1.0f, f16_exp[0]);
+    if constexpr (ESZ == 1) add = offsum[0];
 #pragma unroll
     for (int d = 0; d < DW; d++) {
         const int dm = dm0 + d;
         if (dm >= ndm) break;
         float* o = out + (int64_t)dm * ld_out;
 #pragma unroll
-        for (int j = 0; j < SL; j++) {
-            const int t = t0 + lane + 64 * j;
-            if (t < nt) {
-                if constexpr (sizeof(TS) == 4) o[t] = acc[d][j] / fnch;
-                else if constexpr (sizeof(TS) == 2) o[t] = (acc[d][j] * mul) / fnch;
-                else o[t] = (acc[d][j] + add) / fnch;
+        for (int q = 0; q < NDW; q++) {
+#pragma unroll
+            for (int e = 0; e < EPD; e++) {
+                const int t = t0 + EPD * lane + 64 * EPD * q + e;
+                const float a = acc[d][EPD * q + e];
+                if (t < nt) {
+                    if constexpr (ESZ == 4) o[t] = a / fnch;
+                    else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
+                    else o[t] = (a + add) / fnch;
+                }
             }
         }
     }
@@ -321,14 +367,16 @@ struct TileCfg {
     int sl, dw, w;
     const char* name;
 };
+// samples per lane (multiple of 4 so u8 lanes read whole dwords), DMs per
+// wave, waves per workgroup.  Tile = 64*SL samples x W*DW DMs.
 constexpr TileCfg kTiles[] = {
-    {4, 8, 4, "T256_dm32_w4"},  {4, 16, 4, "T256_dm64_w4"}, {8, 8, 4, "T512_dm32_w4"},
-    {2, 16, 4, "T128_dm64_w4"}, {4, 8, 8, "T256_dm64_w8"},  {8, 16, 4, "T512_dm64_w4"},
-    {2, 8, 8, "T128_dm64_w8"},  {4, 4, 8, "T256_dm32_w8"},
+    {4, 16, 4, "T256_dm64_w4"},   {8, 16, 4, "T512_dm64_w4"},  {8, 8, 4, "T512_dm32_w4"},
+    {8, 8, 8, "T512_dm64_w8"},    {4, 8, 8, "T256_dm64_w8"},   {8, 4, 8, "T512_dm32_w8"},
+    {16, 4, 8, "T1024_dm32_w8"},  {4, 16, 8, "T256_dm128_w8"}, {8, 4, 16, "T512_dm64_w16"},
+    {4, 4, 16, "T256_dm64_w16"},  {4, 8, 16, "T256_dm128_w16"}, {16, 8, 4, "T1024_dm32_w4"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
-constexpr int kDefaultTile = 1;
-constexpr size_t kLdsBudget = 64 * 1024;
+constexpr int kDefaultTile = 3;
 
 template <typename TS, int I>
 const void* tile_kernel_ptr() {
@@ -345,9 +393,14 @@ const void* tile_kernel(int i) {
         case 4: return tile_kernel_ptr<TS, 4>();
         case 5: return tile_kernel_ptr<TS, 5>();
         case 6: return tile_kernel_ptr<TS, 6>();
-        default: return tile_kernel_ptr<TS, 7>();
+        case 7: return tile_kernel_ptr<TS, 7>();
+        case 8: return tile_kernel_ptr<TS, 8>();
+        case 9: return tile_kernel_ptr<TS, 9>();
+        case 10: return tile_kernel_ptr<TS, 10>();
+        default: return tile_kernel_ptr<TS, 11>();
     }
 }
+static_assert(kNumTiles == 12, "tile_kernel() covers 12 configurations");
 
 size_t storage_bytes(int storage) {
     return storage == VFRB_STORE_F32 ? 4 : storage == VFRB_STORE_F16 ? 2 : 1;
@@ -404,7 +457,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     const int T = 64 * tc.sl, DMB = tc.w * tc.dw;
     const int nch = p->nch, ndm = p->ndm;
     const int nblk = (ndm + DMB - 1) / DMB;
-    const size_t esz = storage_bytes(p->storage);
+    size_t max_data = 0;
     std::vector<uint16_t> rel((size_t)nblk * nch * DMB, 0);
     std::vector<int> dmin((size_t)nblk * nch, 0);
     std::vector<Chunk> chunks;
@@ -427,19 +480,23 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
         }
         blk[b].x = (int)chunks.size();
         int c = 0;
+        // A staged channel takes 4*stride bytes in every storage type: one
+        // f32 copy, two shifted f16 copies or four shifted u8 copies.
+        auto stride_of = [&](int s) { return (T + s + 3) / 4 * 4; };
         while (c < nch) {
             int count = 0, smax = 0;
-            while (c + count < nch) {
+            while (c + count < nch && count < kChunkMax) {
                 const int s2 = std::max(smax, span[c + count]);
-                const size_t need = (size_t)(count + 1) * (size_t)(T + s2) * esz;
-                if (need > kLdsBudget || count >= 64) break;
+                const size_t need = (size_t)(count + 1) * 4 * (size_t)stride_of(s2);
+                if (need > kDataBudget) break;
                 smax = s2;
                 count++;
             }
             VFRB_REQUIRE(count > 0, VFRB_ERR_UNSUPPORTED,
                          "one channel's staged span (%d rows) exceeds the LDS budget; use a tile "
                          "with fewer DMs per block", T + span[c]);
-            chunks.push_back(Chunk{c, count, T + smax, 0});
+            chunks.push_back(Chunk{c, count, stride_of(smax), 0});
+            max_data = std::max(max_data, (size_t)count * 4 * stride_of(smax));
             c += count;
         }
         blk[b].y = (int)chunks.size() - blk[b].x;
@@ -459,7 +516,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     }
     p->nblk = nblk;
     p->tile = i;
-    p->lds_bytes = kLdsBudget;
+    p->lds_bytes = (size_t)kChunkMax * DMB * 4 + max_data;
     return VFRB_OK;
 }
 
@@ -641,6 +698,47 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
                     (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out};
     VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->lds_bytes, vfrb::as_stream(stream)));
     return VFRB_OK;
+}
+
+// Autotuner: times every tile configuration that fits this DM list on the
+// window currently loaded and keeps the fastest.
+vfrb_status vfrb_search_autotune(vfrb_search_plan* p, int32_t nrows, float* out, int64_t ld_out,
+                                 int32_t iters, float* ms, void* stream) {
+    VFRB_REQUIRE(p && out && iters > 0, VFRB_ERR_INVALID, "bad autotune arguments");
+    hipStream_t s = vfrb::as_stream(stream);
+    hipEvent_t e0, e1;
+    VFRB_HIP(hipEventCreate(&e0));
+    VFRB_HIP(hipEventCreate(&e1));
+    const int keep = p->tile;
+    int best = -1;
+    float best_ms = 0.0f;
+    vfrb_status st = VFRB_OK;
+    for (int i = 0; i < kNumTiles; i++) {
+        if (ms) ms[i] = -1.0f;
+        if (build_tiles(p, i) != VFRB_OK) continue;  // does not fit this DM list
+        st = vfrb_search_dedisperse(p, nrows, out, ld_out, stream);  // warm-up
+        if (st != VFRB_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { st = VFRB_ERR_HIP; break; }
+        for (int k = 0; k < iters && st == VFRB_OK; k++) st = vfrb_search_dedisperse(p, nrows, out, ld_out, stream);
+        if (st != VFRB_OK) break;
+        float t = 0.0f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            st = VFRB_ERR_HIP;
+            break;
+        }
+        t /= iters;
+        if (ms) ms[i] = t;
+        if (best < 0 || t < best_ms) {
+            best = i;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const vfrb_status st2 = build_tiles(p, best >= 0 ? best : keep);
+    if (st != VFRB_OK) return st;
+    return st2;
 }
 
 vfrb_status vfrb_search_execute(vfrb_search_plan* p, int32_t nrows, const float* fb_tm, float* out,

@@ -116,6 +116,7 @@ _SIGS = {
     "vfrb_search_tile_name": (C.c_char_p, [i32]),
     "vfrb_search_set_tile": (i32, [vp, i32]),
     "vfrb_search_get_tile": (i32, [vp]),
+    "vfrb_search_autotune": (i32, [vp, i32, vp, i64, i32, P(f32), vp]),
     "vfrb_dm_stats": (i32, [vp, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "vfrb_candidates": (i32, [vp, i64, i32, i32, vp, vp, f32, vp, vp, vp, i32, vp, vp]),
     "vfrb_vdif_decode": (i32, [P(C.c_uint8), P(VdifHeader)]),
@@ -448,6 +449,14 @@ class SearchPlan:
 
     def set_tile(self, i):
         call("vfrb_search_set_tile", self.h, i)
+
+    def autotune(self, nrows, out: DeviceArray, ld_out, iters=2, stream=None):
+        """Times every tile configuration on the loaded window, keeps the
+        fastest; returns {tile name: ms} (-1 = does not fit)."""
+        n = lib.vfrb_search_tile_count()
+        ms = np.zeros(n, np.float32)
+        call("vfrb_search_autotune", self.h, nrows, out.ptr, ld_out, iters, _ptr(ms, f32), _s(stream))
+        return dict(zip(tile_names(), ms.tolist()))
 
     @property
     def tile(self):
