@@ -95,18 +95,107 @@ __device__ __forceinline__ void dft5(cf* v) {
     v[2] = cadd(p2, mul_negi(q2));
     v[3] = csub(p2, mul_negi(q2));
 }
+// Twiddle constants w_R^m = exp(-2*pi*i*m/R) for the in-register radix-16
+// and radix-32 DFTs, from quarter-wave cosine tables (exact 0 and 1).
+// cos(2*pi*i/32) for i = 0..8 (a quarter wave); R = 16 uses every other entry.
+__device__ __forceinline__ constexpr float qcos32(int i) {
+    switch (i) {
+        case 0: return 1.0f;
+        case 1: return 0.98078528040323044913f;
+        case 2: return 0.92387953251128675613f;
+        case 3: return 0.83146961230254523708f;
+        case 4: return 0.70710678118654752440f;
+        case 5: return 0.55557023301960222474f;
+        case 6: return 0.38268343236508977173f;
+        case 7: return 0.19509032201612826785f;
+        default: return 0.0f;
+    }
+}
+template <int R>
+__device__ __forceinline__ constexpr float cos_r(int m) {  // cos(2*pi*m/R), R | 32
+    constexpr int Q = R / 4;
+    constexpr int S = 32 / R;
+    m = ((m % R) + R) % R;
+    if (m <= Q) return qcos32(S * m);
+    if (m <= 2 * Q) return -qcos32(S * (2 * Q - m));
+    if (m <= 3 * Q) return -qcos32(S * (m - 2 * Q));
+    return qcos32(S * (4 * Q - m));
+}
+template <int R>
+__device__ __forceinline__ cf wconst(int m) {  // exp(-2*pi*i*m/R)
+    return cf{cos_r<R>(m), -cos_r<R>(m - R / 4)};
+}
+
+template <int R>
+__device__ __forceinline__ void dft(cf* v);
+
+// R = R1*R2 DFT: R1 DFTs of length R2 over n2, twiddles w_R^(n1*k2), R2 DFTs
+// of length R1 over n1; output k = k2 + R2*k1 (natural order).
+template <int R1, int R2>
+__device__ __forceinline__ void dft_split(cf* v) {
+    constexpr int R = R1 * R2;
+    cf a[R1][R2];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) a[n1][n2] = v[n1 + R1 * n2];
+        dft<R2>(a[n1]);
+    }
+#pragma unroll
+    for (int n1 = 1; n1 < R1; n1++)
+#pragma unroll
+        for (int k2 = 1; k2 < R2; k2++) a[n1][k2] = cmul(a[n1][k2], wconst<R>(n1 * k2));
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) {
+        cf b[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) b[n1] = a[n1][k2];
+        dft<R1>(b);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; k1++) v[k2 + R2 * k1] = b[k1];
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void dft(cf* v) {
     if constexpr (R == 2) dft2(v);
     else if constexpr (R == 3) dft3(v);
     else if constexpr (R == 4) dft4(v);
     else if constexpr (R == 5) dft5(v);
-    else dft8(v);
+    else if constexpr (R == 8) dft8(v);
+    else if constexpr (R == 16) dft_split<4, 4>(v);
+    else dft_split<4, 8>(v);
+}
+
+// Multiplies v[r] (r = 1..R-1) by w^r, where w^(2^b) = tw[(2^b)*idx] come
+// from the FP64-computed table and the other powers are products of those
+// (at most log2(R)-1 multiplications deep).
+template <int R>
+__device__ __forceinline__ void apply_twiddles(cf* v, const float2* __restrict__ tw, int idx) {
+    cf w[R];
+#pragma unroll
+    for (int b = 1; b < R; b <<= 1) {
+        const float2 t = tw[b * idx];
+        w[b] = cf{t.x, t.y};
+    }
+#pragma unroll
+    for (int r = 3; r < R; r++) {
+        if ((r & (r - 1)) == 0) continue;
+        const int low = r & -r;
+        w[r] = cmul(w[r - low], w[low]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r]);
 }
 
 // ---------------------------------------------------------------------------
-// Power-of-two kernel: 32 complex elements per thread, N/32 threads per FFT,
-// BLOCK/(N/32) FFTs of the same output row in flight per workgroup.
+// Power-of-two kernel.  32 complex elements per thread, TPF = N/32 threads per
+// FFT, FPB = BLOCK/TPF FFTs of the same output row in flight per workgroup.
+// Pass plan: radix 8 straight from the byte payload (4 consecutive
+// butterflies per thread, dword loads), radix-R middle passes through LDS,
+// and a final radix-16 pass in which thread t owns the butterflies t and
+// N/16 - t, so Z[k] and Z[N-k] meet in registers and the detected power
+// never goes back to LDS.
 // ---------------------------------------------------------------------------
 template <int N>
 struct P2 {
@@ -115,11 +204,14 @@ struct P2 {
     static constexpr int FPB = BLOCK / TPF;
     static constexpr int TOT = FPB * N;
     static constexpr int LDS_WORDS = TOT + TOT / 32;      // per component
-    static constexpr int CPT = (N / 2 + BLOCK - 1) / BLOCK;  // output channels per thread
     static constexpr int LOG2 = __builtin_ctz(N);
+    static constexpr int MID_BITS = LOG2 - 7;             // between radix-8 and radix-16
+    // middle radices (0 = unused): 8192 -> 8,8; 16384 -> 16,8; else one pass
+    static constexpr int MID0 = MID_BITS == 6 ? 8 : MID_BITS == 7 ? 16 : (1 << MID_BITS);
+    static constexpr int MID1 = MID_BITS == 6 ? 8 : MID_BITS == 7 ? 8 : 0;
 };
 
-// One radix-R Stockham pass of size-N FFTs; t = thread index within its FFT,
+// One radix-R Stockham pass through LDS; t = thread index within its FFT,
 // fb = element offset of that FFT in LDS.
 template <int N, int R>
 __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restrict__ im, int Ns, int t,
@@ -143,28 +235,7 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
     for (int q = 0; q < Q; q++) {
         const int j = t + TPF * q;
         const int k = j & (Ns - 1);
-        if (Ns > 1) {
-            const float2 w = tw[k << tw_shift];
-            const cf w1 = {w.x, w.y};
-            if constexpr (R == 2) {
-                v[q][1] = cmul(v[q][1], w1);
-            } else if constexpr (R == 4) {
-                const cf w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-                v[q][1] = cmul(v[q][1], w1);
-                v[q][2] = cmul(v[q][2], w2);
-                v[q][3] = cmul(v[q][3], w3);
-            } else {
-                const cf w2 = cmul(w1, w1), w3 = cmul(w2, w1), w4 = cmul(w2, w2);
-                const cf w5 = cmul(w4, w1), w6 = cmul(w3, w3), w7 = cmul(w4, w3);
-                v[q][1] = cmul(v[q][1], w1);
-                v[q][2] = cmul(v[q][2], w2);
-                v[q][3] = cmul(v[q][3], w3);
-                v[q][4] = cmul(v[q][4], w4);
-                v[q][5] = cmul(v[q][5], w5);
-                v[q][6] = cmul(v[q][6], w6);
-                v[q][7] = cmul(v[q][7], w7);
-            }
-        }
+        apply_twiddles<R>(v[q], tw, k << tw_shift);
         dft<R>(v[q]);
         const int base = (j - k) * R + k;
 #pragma unroll
@@ -177,87 +248,166 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
     __syncthreads();
 }
 
+__device__ __forceinline__ float norm2(cf a) { return a.x * a.x + a.y * a.y; }
+
+// LDS (66 KB for N <= 8192) allows two workgroups per CU, i.e. 2 waves per
+// SIMD: hold the register allocator to that budget.
 template <int N>
-__global__ void __launch_bounds__(P2<N>::BLOCK)
+__global__ void __launch_bounds__(P2<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
 fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                const float2* __restrict__ tw, float* __restrict__ fb, int64_t nrows, int nave,
                int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale) {
     using C = P2<N>;
+    constexpr int TPF = C::TPF;
+    constexpr int NS = N / 16;      // butterflies of the last pass
+    constexpr int EIGHTH = N / 8;   // input stride of the first pass
     extern __shared__ float lds[];
     float* re = lds;
     float* im = lds + C::LDS_WORDS;
     const int tid = threadIdx.x;
     const int64_t row = blockIdx.x;
     if (row >= nrows) return;
-    const int fslot = tid / C::TPF;  // FFT slot of this thread
-    const int t = tid % C::TPF;
-    const int fbase = fslot * N;
+    const int slot = tid / TPF;
+    const int t = tid % TPF;
+    const int fbase = slot * N;
 
-    float acc[C::CPT];
+    // Last-pass butterflies of this thread and their twiddles (frame invariant).
+    const int ja = t == 0 ? 0 : t;
+    const int jb = t == 0 ? N / 32 : NS - t;
+
+    float acc[16];
 #pragma unroll
-    for (int i = 0; i < C::CPT; i++) acc[i] = 0.0f;
+    for (int i = 0; i < 16; i++) acc[i] = 0.0f;
 
-    const int e0 = tid * 32;  // first element this thread unpacks
-    const int lframe = e0 / N;
-    for (int it = 0; it < nave; it += C::FPB) {
-        // ---- unpack 32 samples per pol: z = (x0-128) + i(x1-128)
-        const int frame = it + lframe;
-        uint4 a0 = {0, 0, 0, 0}, a1 = {0, 0, 0, 0}, b0 = {0, 0, 0, 0}, b1 = {0, 0, 0, 0};
+    // prefetch registers: 8 dwords per pol = samples 4t..4t+3 + r*N/8
+    uint32_t x0[8], x1[8];
+    auto fetch = [&](int it) {
+        const int frame = it + slot;
         if (frame < nave) {
-            const size_t off = ((size_t)row * nave + frame) * N + (e0 % N);
-            const uint4* p0 = reinterpret_cast<const uint4*>(pol0 + off);
-            const uint4* p1 = reinterpret_cast<const uint4*>(pol1 + off);
-            a0 = p0[0];
-            a1 = p0[1];
-            b0 = p1[0];
-            b1 = p1[1];
+            const uint8_t* b0 = pol0 + ((size_t)row * nave + frame) * N + 4 * t;
+            const uint8_t* b1 = pol1 + ((size_t)row * nave + frame) * N + 4 * t;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                x0[r] = *reinterpret_cast<const uint32_t*>(b0 + r * EIGHTH);
+                x1[r] = *reinterpret_cast<const uint32_t*>(b1 + r * EIGHTH);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) x0[r] = x1[r] = 0x80808080u;  // 128 -> 0
         }
-        const uint32_t w0[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-        const uint32_t w1[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-        const bool live = frame < nave;
+    };
+    fetch(0);
+    for (int it = 0; it < nave; it += C::FPB) {
+        // The twiddles depend only on the thread index; an opaque copy of the
+        // table pointer per frame keeps the compiler from hoisting all of them
+        // out of the loop into (spilled) registers.  The asm has no memory
+        // access: it only hides the pointer's value.
+        const float2* twf = tw;
+        asm volatile("" : "+s"(twf));
+        // ---- pass 1: radix 8, butterflies 4t+q, inputs z[4t+q + r*N/8]
+        cf v[4][8];
 #pragma unroll
-        for (int w = 0; w < 8; w++) {
+        for (int r = 0; r < 8; r++) {
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int idx = lds_pad(e0 + w * 4 + b);
-                re[idx] = live ? (float)((w0[w] >> (8 * b)) & 0xffu) - 128.0f : 0.0f;
-                im[idx] = live ? (float)((w1[w] >> (8 * b)) & 0xffu) - 128.0f : 0.0f;
+            for (int q = 0; q < 4; q++) {
+                v[q][r] = cf{(float)((x0[r] >> (8 * q)) & 0xffu) - 128.0f,
+                             (float)((x1[r] >> (8 * q)) & 0xffu) - 128.0f};
+            }
+        }
+        if (it + C::FPB < nave) fetch(it + C::FPB);  // next frame's bytes in flight
+#pragma unroll
+        for (int q = 0; q < 4; q++) dft8(v[q]);
+        __syncthreads();  // previous frame's last pass has read LDS
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int idx = lds_pad(fbase + 32 * t + 8 * q + r);
+                re[idx] = v[q][r].x;
+                im[idx] = v[q][r].y;
             }
         }
         __syncthreads();
-        // ---- FFT: radix-8 passes, then a radix-2 or radix-4 pass
-        int Ns = 1;
-#pragma unroll
-        for (int p = 0; p < C::LOG2 / 3; p++) {
-            p2_pass<N, 8>(re, im, Ns, t, fbase, tw);
-            Ns *= 8;
+        // ---- middle passes
+        int Ns = 8;
+        if constexpr (C::MID0 > 1) {
+            p2_pass<N, C::MID0>(re, im, Ns, t, fbase, twf);
+            Ns *= C::MID0;
         }
-        if constexpr (C::LOG2 % 3 == 1) p2_pass<N, 2>(re, im, Ns, t, fbase, tw);
-        if constexpr (C::LOG2 % 3 == 2) p2_pass<N, 4>(re, im, Ns, t, fbase, tw);
-        // ---- detect: |Z[k]|^2 + |Z[N-k]|^2 summed over the FFTs in flight
+        if constexpr (C::MID1 > 1) {
+            p2_pass<N, C::MID1>(re, im, Ns, t, fbase, twf);
+            Ns *= C::MID1;
+        }
+        // ---- last pass: radix 16 on butterflies ja, jb (Ns == N/16); only
+        // |Z|^2 is needed: na[r] = |Z[ja + r*NS]|^2, nb[r] = |Z[jb + r*NS]|^2
+        float na[16], nb[16];
+        {
+            cf a[16];
 #pragma unroll
-        for (int i = 0; i < C::CPT; i++) {
-            const int c = tid + C::BLOCK * i;
-            if (c < nchanout) {
-                const int k = nskip + c;
-                float s = 0.0f;
+            for (int r = 0; r < 16; r++) {
+                const int ia = lds_pad(fbase + ja + r * NS);
+                a[r] = cf{re[ia], im[ia]};
+            }
+            apply_twiddles<16>(a, twf, ja);
+            dft<16>(a);
 #pragma unroll
-                for (int f = 0; f < C::FPB; f++) {
-                    const int ia = lds_pad(f * N + k), ib = lds_pad(f * N + ((N - k) & (N - 1)));
-                    s += re[ia] * re[ia] + im[ia] * im[ia] + re[ib] * re[ib] + im[ib] * im[ib];
-                }
-                acc[i] += s;
+            for (int r = 0; r < 16; r++) na[r] = norm2(a[r]);
+        }
+        {
+            cf b[16];
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int ib = lds_pad(fbase + jb + r * NS);
+                b[r] = cf{re[ib], im[ib]};
+            }
+            apply_twiddles<16>(b, twf, jb);
+            dft<16>(b);
+#pragma unroll
+            for (int r = 0; r < 16; r++) nb[r] = norm2(b[r]);
+        }
+        // ---- detect: |Z[k]|^2 + |Z[N-k]|^2
+        if (t != 0) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                acc[r] += na[r] + nb[15 - r];      // k = ja + r*NS
+                acc[8 + r] += nb[r] + na[15 - r];  // k = jb + r*NS
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                acc[r] += nb[r] + nb[15 - r];          // k = N/32 + r*NS
+                acc[8 + r] += na[r + 1] + na[15 - r];  // k = (r+1)*NS
             }
         }
-        __syncthreads();
     }
+    // ---- sum the FFT slots of this row, then write the selected channels
+    if constexpr (C::FPB > 1) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 16; i++) re[i * C::BLOCK + tid] = acc[i];
+        __syncthreads();
+        if (slot == 0) {
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                float s = acc[i];
+#pragma unroll
+                for (int f = 1; f < C::FPB; f++) s += re[i * C::BLOCK + f * TPF + t];
+                acc[i] = s;
+            }
+        }
+    }
+    if (slot != 0) return;
     int64_t orow = row0 + row;
     if (ring_rows > 0) orow %= ring_rows;
     float* dst = fb + orow * (int64_t)nchanout;
 #pragma unroll
-    for (int i = 0; i < C::CPT; i++) {
-        const int c = tid + C::BLOCK * i;
-        if (c < nchanout) dst[c] = acc[i] * scale;
+    for (int i = 0; i < 16; i++) {
+        const int r = i & 7;
+        int k;
+        if (t != 0) k = (i < 8 ? ja : jb) + r * NS;
+        else k = i < 8 ? N / 32 + r * NS : (r + 1) * NS;
+        const int c = k - nskip;
+        if (c >= 0 && c < nchanout) dst[c] = acc[i] * scale;
     }
 }
 
