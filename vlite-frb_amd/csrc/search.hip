@@ -201,7 +201,8 @@ constexpr h2_t kSelLo = {(_Float16)1.0f, (_Float16)0.0f};
 constexpr h2_t kSelHi = {(_Float16)0.0f, (_Float16)1.0f};
 
 struct Chunk {
-    int c0, count, stride, pad;  // stride: staged samples per copy, multiple of 4
+    int c0, count, stride;  // stride: staged samples per copy, multiple of 4
+    int roff;               // first entry of this chunk in the LDS byte-offset table
 };
 
 // At most kChunkMax channels per LDS chunk; staged rows take at most
@@ -209,6 +210,7 @@ struct Chunk {
 // and two fit in one CU's 160 KB of LDS.
 constexpr int kChunkMax = 32;
 constexpr size_t kDataBudget = 60 * 1024;
+constexpr int kStageU = 8;  // staging loads in flight per lane
 
 // LDS (<= 76 KB per workgroup) limits a CU to two workgroups, i.e. W/2 waves
 // per SIMD: let the register allocator use the VGPR budget that occupancy
@@ -222,8 +224,8 @@ struct WavesPerEu {
 template <typename TS, int SL, int DW, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
-              const uint16_t* __restrict__ rel,  // [nblk][nch][W*DW]
-              const int* __restrict__ dmin,      // [nblk][nch]
+              const uint32_t* __restrict__ roffs,  // per chunk: [count][W*DW] LDS byte offsets
+              const int* __restrict__ dmin,        // [nblk][nch]
               const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
               const float* __restrict__ chan_scale,  // u8 only
               const int* __restrict__ f16_exp, const float* __restrict__ offsum,
@@ -257,29 +259,40 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     for (int k = 0; k < cr.y; k++) {
         const Chunk ch = chunks[cr.x + k];
         const int rowb = 4 * ch.stride;    // bytes per staged channel (all copies)
-        // stage copy p: element i = window[c][t0 + dmin_c + i + p], rows >= nrows -> 0
+        const int per_row = ch.stride + EPD - 1;
+        // stage copy p: element i = window[c][t0 + dmin_c + i + p], rows >= nrows -> 0;
+        // kStageU loads per lane are in flight before the LDS writes
         for (int ci = wave; ci < ch.count; ci += W) {
             const int c = ch.c0 + ci;
             const int64_t src0 = (int64_t)c * ld;
             const int r0 = t0 + dmin[(int64_t)b * nch + c];
             unsigned char* row = data + ci * rowb;
-            for (int s = lane; s < ch.stride + EPD - 1; s += 64) {
-                const int r = r0 + s;
-                const TS v = r < nrows ? win[src0 + r] : TS(0);
+            for (int s0 = 0; s0 < per_row; s0 += 64 * kStageU) {
+                TS v[kStageU];
 #pragma unroll
-                for (int p = 0; p < EPD; p++) {
-                    const int i = s - p;
-                    if (i >= 0 && i < ch.stride)
-                        reinterpret_cast<TS*>(row + p * ch.stride * ESZ)[i] = v;
+                for (int u = 0; u < kStageU; u++) {
+                    const int s = s0 + lane + 64 * u;
+                    const int r = r0 + s;
+                    v[u] = (s < per_row && r < nrows) ? win[src0 + r] : TS(0);
+                }
+#pragma unroll
+                for (int u = 0; u < kStageU; u++) {
+                    const int s = s0 + lane + 64 * u;
+#pragma unroll
+                    for (int p = 0; p < EPD; p++) {
+                        const int i = s - p;
+                        if (i >= 0 && i < ch.stride)
+                            reinterpret_cast<TS*>(row + p * ch.stride * ESZ)[i] = v[u];
+                    }
                 }
             }
         }
-        // byte offset of the first dword of each (channel, DM) read
-        for (int idx = threadIdx.x; idx < ch.count * DMB; idx += 64 * W) {
-            const int ci = idx / DMB, d = idx - ci * DMB;
-            const int r = rel[((int64_t)b * nch + ch.c0 + ci) * DMB + d];
-            const int p = r & (EPD - 1);
-            roff[ci * DMB + d] = (uint32_t)(ci * rowb + p * ch.stride * ESZ + (r - p) * ESZ);
+        // byte offset of the first dword of each (channel, DM) read, built on
+        // the host: a contiguous copy of count*DMB words (DMB % 4 == 0)
+        {
+            const uint4* src = reinterpret_cast<const uint4*>(roffs + ch.roff);
+            uint4* dst = reinterpret_cast<uint4*>(roff);
+            for (int i = threadIdx.x; i < ch.count * DMB / 4; i += 64 * W) dst[i] = src[i];
         }
         __syncthreads();
         for (int ci = 0; ci < ch.count; ci++) {
@@ -427,7 +440,7 @@ struct vfrb_search_plan {
     float* d_u8_offsum = nullptr;
     float* d_tmp_fb = nullptr;  // time-major staging for vfrb_search_execute
     // tile tables
-    uint16_t* d_rel = nullptr;
+    uint32_t* d_roffs = nullptr;
     int* d_dmin = nullptr;
     Chunk* d_chunks = nullptr;
     int2* d_blk_chunks = nullptr;
@@ -439,29 +452,36 @@ struct vfrb_search_plan {
 namespace {
 
 void free_tiles(vfrb_search_plan* p) {
-    (void)hipFree(p->d_rel);
+    (void)hipFree(p->d_roffs);
     (void)hipFree(p->d_dmin);
     (void)hipFree(p->d_chunks);
     (void)hipFree(p->d_blk_chunks);
-    p->d_rel = nullptr;
+    p->d_roffs = nullptr;
     p->d_dmin = nullptr;
     p->d_chunks = nullptr;
     p->d_blk_chunks = nullptr;
 }
 
 // Builds the per-DM-block tables for tile configuration i: minimum delay per
-// channel, delays relative to it (u16), and channel chunks whose staged
-// spans fit the LDS budget.
+// channel, channel chunks whose staged rows fit the LDS budget, and for every
+// (chunk channel, DM of the block) the LDS byte offset of the first dword the
+// kernel reads: channel row + copy (delay mod samples-per-dword) + aligned
+// position of the delay inside that copy.
 vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     const TileCfg& tc = kTiles[i];
     const int T = 64 * tc.sl, DMB = tc.w * tc.dw;
     const int nch = p->nch, ndm = p->ndm;
     const int nblk = (ndm + DMB - 1) / DMB;
+    const int esz = (int)storage_bytes(p->storage);
+    const int epd = 4 / esz;
     size_t max_data = 0;
-    std::vector<uint16_t> rel((size_t)nblk * nch * DMB, 0);
     std::vector<int> dmin((size_t)nblk * nch, 0);
     std::vector<Chunk> chunks;
+    std::vector<uint32_t> roffs;
     std::vector<int2> blk(nblk);
+    // A staged channel takes 4*stride bytes in every storage type: one f32
+    // copy, two shifted f16 copies or four shifted u8 copies.
+    auto stride_of = [&](int s) { return (T + s + 3) / 4 * 4; };
     for (int b = 0; b < nblk; b++) {
         const int d0 = b * DMB, d1 = std::min(ndm, d0 + DMB);
         std::vector<int> span(nch);
@@ -471,18 +491,11 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
                 lo = std::min(lo, p->off[(size_t)d * nch + c]);
                 hi = std::max(hi, p->off[(size_t)d * nch + c]);
             }
-            VFRB_REQUIRE(hi - lo < 65536, VFRB_ERR_UNSUPPORTED,
-                         "delay spread %d in a DM block exceeds 65535 rows", hi - lo);
             dmin[(size_t)b * nch + c] = lo;
             span[c] = hi - lo;
-            for (int d = d0; d < d1; d++)
-                rel[((size_t)b * nch + c) * DMB + (d - d0)] = (uint16_t)(p->off[(size_t)d * nch + c] - lo);
         }
         blk[b].x = (int)chunks.size();
         int c = 0;
-        // A staged channel takes 4*stride bytes in every storage type: one
-        // f32 copy, two shifted f16 copies or four shifted u8 copies.
-        auto stride_of = [&](int s) { return (T + s + 3) / 4 * 4; };
         while (c < nch) {
             int count = 0, smax = 0;
             while (c + count < nch && count < kChunkMax) {
@@ -495,18 +508,27 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
             VFRB_REQUIRE(count > 0, VFRB_ERR_UNSUPPORTED,
                          "one channel's staged span (%d rows) exceeds the LDS budget; use a tile "
                          "with fewer DMs per block", T + span[c]);
-            chunks.push_back(Chunk{c, count, stride_of(smax), 0});
-            max_data = std::max(max_data, (size_t)count * 4 * stride_of(smax));
+            const int stride = stride_of(smax);
+            chunks.push_back(Chunk{c, count, stride, (int)roffs.size()});
+            for (int ci = 0; ci < count; ci++) {
+                const int cc = c + ci;
+                for (int d = 0; d < DMB; d++) {
+                    const int r = d0 + d < d1 ? p->off[(size_t)(d0 + d) * nch + cc] - dmin[(size_t)b * nch + cc] : 0;
+                    const int q = r & (epd - 1);
+                    roffs.push_back((uint32_t)(ci * 4 * stride + q * stride * esz + (r - q) * esz));
+                }
+            }
+            max_data = std::max(max_data, (size_t)count * 4 * stride);
             c += count;
         }
         blk[b].y = (int)chunks.size() - blk[b].x;
     }
     free_tiles(p);
-    hipError_t e = hipMalloc(&p->d_rel, rel.size() * sizeof(uint16_t));
+    hipError_t e = hipMalloc(&p->d_roffs, roffs.size() * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&p->d_dmin, dmin.size() * sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&p->d_chunks, chunks.size() * sizeof(Chunk));
     if (e == hipSuccess) e = hipMalloc(&p->d_blk_chunks, blk.size() * sizeof(int2));
-    if (e == hipSuccess) e = hipMemcpy(p->d_rel, rel.data(), rel.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_roffs, roffs.data(), roffs.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_dmin, dmin.data(), dmin.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->d_blk_chunks, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice);
@@ -693,7 +715,7 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
     int64_t ld = p->ld;
     int nch = p->nch, ndm = p->ndm;
     void* args[] = {(void*)&win, (void*)&ld, (void*)&nch, (void*)&nrows, (void*)&nt,
-                    (void*)&ndm, (void*)&p->d_rel, (void*)&p->d_dmin, (void*)&p->d_chunks,
+                    (void*)&ndm, (void*)&p->d_roffs, (void*)&p->d_dmin, (void*)&p->d_chunks,
                     (void*)&p->d_blk_chunks, (void*)&p->d_u8_scale, (void*)&p->d_f16_exp,
                     (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out};
     VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->lds_bytes, vfrb::as_stream(stream)));
