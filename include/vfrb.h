@@ -279,6 +279,53 @@ vfrb_status vfrb_search_autotune(vfrb_search_plan* plan, int32_t nrows, float* o
                                  int64_t ld_out, int32_t iters, float* ms, void* stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched dedispersion (no reference counterpart: the reference runs one    */
+/* dedisp_execute per window, PIPELINE:671)                                  */
+/* ------------------------------------------------------------------------ */
+/*
+ * A batch dedisperses the loaded windows of many search plans (one per
+ * antenna stream: own DM list, own window length) in ONE persistent launch
+ * that pulls (window, DM block, time tile) work items from a list in device
+ * memory.  Every member plan must have the batch's storage type; the batch
+ * moves them to its tile configuration (vfrb_search_set_tile).  Each job's
+ * output equals vfrb_search_dedisperse on that plan alone, bit for bit.
+ */
+typedef struct vfrb_batch vfrb_batch;
+
+vfrb_status vfrb_batch_create(vfrb_batch** batch, int32_t storage, int32_t tile_config,
+                              int32_t max_jobs, int32_t max_tiles);
+vfrb_status vfrb_batch_destroy(vfrb_batch* batch);
+/*
+ * Sets the jobs of the next launch: job i dedisperses the first nrows[i] rows
+ * of plans[i]'s loaded window into outs[i] (f32 [ndm][ld_outs[i]]).  The
+ * work list is copied to the device on `stream`.  *ntiles (optional) receives
+ * the number of work items.  Returns VFRB_ERR_CAPACITY beyond max_jobs or
+ * max_tiles.
+ */
+vfrb_status vfrb_batch_set(vfrb_batch* batch, int32_t njobs, vfrb_search_plan* const* plans,
+                           const int32_t* nrows, float* const* outs, const int64_t* ld_outs,
+                           int32_t* ntiles, void* stream);
+/* One eager persistent launch over the current jobs. */
+vfrb_status vfrb_batch_dedisperse(vfrb_batch* batch, void* stream);
+/*
+ * hipGraph path: the first call captures the persistent launch on `stream`
+ * (a stream from vfrb_stream_create, not the null stream) and instantiates
+ * it; every call replays it.  The graph reads the work list from device
+ * memory, so it stays valid across vfrb_batch_set calls; changing the tile
+ * configuration drops it and the next call captures again.
+ */
+vfrb_status vfrb_batch_graph_launch(vfrb_batch* batch, void* stream);
+/*
+ * Autotuner over the batch: times every tile configuration that fits all
+ * member plans on the current jobs (`iters` launches each), keeps the
+ * fastest and rebuilds the work list for it.  ms as in vfrb_search_autotune.
+ */
+vfrb_status vfrb_batch_autotune(vfrb_batch* batch, int32_t iters, float* ms, void* stream);
+int32_t     vfrb_batch_get_tile(const vfrb_batch* batch);
+/* Persistent workgroups of one launch (occupancy x compute units). */
+int32_t     vfrb_batch_grid(const vfrb_batch* batch);
+
+/* ------------------------------------------------------------------------ */
 /* Per-DM statistics and candidates                                         */
 /* ------------------------------------------------------------------------ */
 /*

@@ -1,0 +1,135 @@
+"""GPU: batched persistent dedispersion (vfrb_batch_*) against the same plans
+run one at a time, the oracle, and hipGraph replays."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NCH = 1024
+TCHAN = 64 * 2048 / 128e6
+CHANBW = 64.0 / NCH
+FCH1 = 320.0 + NCH * CHANBW - 0.5 * CHANBW
+
+# uneven windows and DM ranges: (rows, ndm, max DM)
+JOBS = [(976, 256, 25.5), (520, 64, 6.3), (800, 200, 19.9), (450, 32, 3.1), (700, 128, 12.7)]
+
+
+def _make_jobs(vfrb, storage, seed):
+    rng = np.random.default_rng(seed)
+    jobs, fbs = [], []
+    for rows, ndm, dmax in JOBS:
+        dms = np.linspace(0.0, dmax, ndm).astype(np.float32)
+        plan = vfrb.SearchPlan(NCH, TCHAN, FCH1, CHANBW, dms, rows, vfrb.STORAGE_NAMES[storage])
+        fb = rng.normal(256, 22, (rows, NCH)).astype(np.float32)
+        d = vfrb.DeviceArray.from_host(fb)
+        plan.load(d, rows, bandpass=True)
+        nt = rows - plan.max_delay
+        out = vfrb.DeviceArray((ndm, nt + 7), np.float32)  # ld_out > nt
+        jobs.append((plan, rows, out, nt + 7))
+        fbs.append((fb, d))
+    return jobs, fbs
+
+
+def _singles(vfrb, jobs):
+    res = []
+    for plan, rows, out, ld in jobs:
+        o = vfrb.DeviceArray(out.shape, np.float32)
+        o.zero()
+        plan.dedisperse(rows, o, ld)
+        res.append(o.download()[:, : rows - plan.max_delay])
+    return res
+
+
+def _batch_results(jobs):
+    return [out.download()[:, : rows - plan.max_delay] for plan, rows, out, _ in jobs]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_batch_equals_single_plans(gpu, orc, storage):
+    vfrb = gpu
+    print(f"[case] batch vs single plans storage={storage}", flush=True)
+    jobs, fbs = _make_jobs(vfrb, storage, seed=11)
+    batch = vfrb.Batch(vfrb.STORAGE_NAMES[storage])
+    ntiles = batch.set(jobs)
+    assert ntiles > 0 and batch.grid > 0
+    for _, _, out, _ in jobs:
+        out.zero()
+    batch.dedisperse()
+    vfrb.sync()
+    got = _batch_results(jobs)
+    # same tile configuration and summation order: bit-identical to one plan alone
+    for g, s in zip(got, _singles(vfrb, jobs)):
+        assert np.array_equal(g, s)
+    if storage == "f32":
+        for (plan, rows, _, _), (fb, _), g in zip(jobs, fbs, got):
+            ref = orc.dedisperse_f32(orc.bandpass_f32(fb), plan.offsets(), rows - plan.max_delay)
+            assert np.max(np.abs(g - ref)) < 2e-3
+
+
+def test_batch_graph_replays_match_eager(gpu):
+    vfrb = gpu
+    print("[case] batch graph replay", flush=True)
+    jobs, fbs = _make_jobs(vfrb, "f16", seed=12)
+    batch = vfrb.Batch(vfrb.STORE_F16)
+    s = vfrb.Stream()
+    batch.set(jobs, stream=s)
+    batch.dedisperse(stream=s)
+    s.sync()
+    eager = _batch_results(jobs)
+    for k in range(3):
+        for _, _, out, _ in jobs:
+            out.zero()
+        batch.graph_launch(s)
+        s.sync()
+        for g, e in zip(_batch_results(jobs), eager):
+            assert np.array_equal(g, e), f"replay {k}"
+        _singles(vfrb, jobs)  # eager work between replays
+    # new job set (fewer, shorter windows) through the same captured graph
+    jobs2 = [(p, r - 100, o, ld) for p, r, o, ld in jobs[:3]]
+    batch.set(jobs2, stream=s)
+    for _, _, out, _ in jobs2:
+        out.zero()
+    batch.graph_launch(s)
+    s.sync()
+    for g, e in zip(_batch_results(jobs2), _singles(vfrb, jobs2)):
+        assert np.array_equal(g, e)
+
+
+def test_batch_autotune_keeps_results(gpu):
+    vfrb = gpu
+    print("[case] batch autotune", flush=True)
+    jobs, _ = _make_jobs(vfrb, "f32", seed=13)
+    batch = vfrb.Batch(vfrb.STORE_F32)
+    batch.set(jobs)
+    ms = batch.autotune(iters=1)
+    names = vfrb.tile_names()
+    assert batch.tile >= 0 and ms[names[batch.tile]] == min(v for v in ms.values() if v > 0)
+    for _, _, out, _ in jobs:
+        out.zero()
+    batch.dedisperse()
+    vfrb.sync()
+    for plan, *_ in jobs:
+        assert plan.tile == batch.tile
+    for g, s in zip(_batch_results(jobs), _singles(vfrb, jobs)):
+        assert np.array_equal(g, s)
+
+
+def test_batch_rejects_bad_jobs(gpu):
+    vfrb = gpu
+    jobs, _ = _make_jobs(vfrb, "f32", seed=14)
+    small = vfrb.Batch(vfrb.STORE_F32, max_jobs=2, max_tiles=8)
+    with pytest.raises(vfrb.VfrbError) as e:
+        small.set(jobs)
+    assert e.value.status == vfrb.ERR_CAPACITY
+    with pytest.raises(vfrb.VfrbError) as e:
+        small.set(jobs[:2])  # two jobs fit, their work items do not
+    assert e.value.status == vfrb.ERR_CAPACITY
+    other = vfrb.Batch(vfrb.STORE_F16)
+    with pytest.raises(vfrb.VfrbError):
+        other.set(jobs[:1])  # storage mismatch
+    b = vfrb.Batch(vfrb.STORE_F32)
+    plan, rows, out, ld = jobs[0]
+    with pytest.raises(vfrb.VfrbError):
+        b.set([(plan, rows + 1, out, ld)])  # more rows than loaded
+    with pytest.raises(vfrb.VfrbError):
+        b.dedisperse()  # no jobs set after the failure

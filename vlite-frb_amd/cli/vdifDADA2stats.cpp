@@ -138,7 +138,7 @@ int main(int argc, char** argv) {
         std::fprintf(flog, "initframe01 returned in error!\n");
         return 2;
     }
-    int loopcnt = 0, nwin = 0;
+    int loopcnt = 0;
     std::fprintf(flog, "begin main loop...\n");
     for (;;) {
         if (vfrb_seq_fill(seq, (int32_t)nloop, h0, h1) != VFRB_OK) break;
@@ -160,7 +160,6 @@ int main(int argc, char** argv) {
                 std::fclose(f);
             }
             loopcnt = 0;
-            nwin++;
         }
     }
     std::fprintf(flog, "Looped %d times\n\n", loopcnt);

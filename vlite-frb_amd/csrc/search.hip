@@ -220,21 +220,22 @@ struct WavesPerEu {
     static constexpr int value = W / 2 > 1 ? W / 2 : 1;
 };
 
-// SL samples per lane, DW DMs per wave, W waves; T = 64*SL samples per tile.
+// One tile: DM block b (W*DW DMs) x time tile tx (T = 64*SL samples) of one
+// window.  SL samples per lane, DW DMs per wave, W waves.  The tile ends
+// with a barrier, so the caller may reuse the LDS at once.
 template <typename TS, int SL, int DW, int W>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
-dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
-              const uint32_t* __restrict__ roffs,  // per chunk: [count][W*DW] LDS byte offsets
-              const int* __restrict__ dmin,        // [nblk][nch]
-              const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
-              const float* __restrict__ chan_scale,  // u8 only
-              const int* __restrict__ f16_exp, const float* __restrict__ offsum,
-              float* __restrict__ out, int64_t ld_out) {
+__device__ __forceinline__ void
+dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+            const uint32_t* __restrict__ roffs,  // per chunk: [count][W*DW] LDS byte offsets
+            const int* __restrict__ dmin,        // [nblk][nch]
+            const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+            const float* __restrict__ chan_scale,  // u8 only
+            const int* __restrict__ f16_exp, const float* __restrict__ offsum,
+            float* __restrict__ out, int64_t ld_out, int b, int tx, unsigned char* lds_raw) {
     // LDS: [kChunkMax][DMB] u32 byte offsets of each (channel, DM) read,
     // then the staged rows.  A row holds COPIES copies of the channel's
     // samples, copy p shifted by p elements, so that the dword a lane reads
     // (EPD consecutive samples) is always 4-byte aligned whatever the delay.
-    extern __shared__ unsigned char lds_raw[];
     constexpr int ESZ = (int)sizeof(TS);
     constexpr int EPD = 4 / ESZ;           // samples per dword = copies per row
     constexpr int T = 64 * SL;
@@ -245,8 +246,7 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     unsigned char* data = lds_raw + kChunkMax * DMB * 4;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int b = blockIdx.y;
-    const int t0 = blockIdx.x * T;
+    const int t0 = tx * T;
     const int dm0 = b * DMB + wave * DW;
     const int2 cr = blk_chunks[b];
 
@@ -350,6 +350,75 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     }
 }
 
+// One workgroup per tile: grid = (time tiles, DM blocks).
+template <typename TS, int SL, int DW, int W>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
+dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+              const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
+              const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+              const float* __restrict__ chan_scale, const int* __restrict__ f16_exp,
+              const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out) {
+    extern __shared__ unsigned char lds_raw[];
+    dedisp_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
+                               chan_scale, f16_exp, offsum, out, ld_out, blockIdx.y, blockIdx.x,
+                               lds_raw);
+}
+
+// ---------------------------------------------------------------------------
+// Batched dedispersion: one persistent launch over the tiles of many windows
+// (jobs), each with its own DM list, length and storage scales.  The work
+// list lives in HBM, so a captured launch can be replayed on new jobs.
+// ---------------------------------------------------------------------------
+struct BatchJob {
+    const void* win;
+    int64_t ld;
+    int nch, nrows, nt, ndm;
+    const uint32_t* roffs;
+    const int* dmin;
+    const Chunk* chunks;
+    const int2* blk_chunks;
+    const float* chan_scale;
+    const int* f16_exp;
+    const float* offsum;
+    float* out;
+    int64_t ld_out;
+};
+
+struct BatchTile {
+    int job, blk, tx, pad;
+};
+
+// ctr[0]: next tile, ctr[1]: workgroups done.  The last workgroup to finish
+// puts both back to zero, so the next launch (or graph replay) starts clean
+// without a memset; ntiles[0] = tiles in the list.
+template <typename TS, int SL, int DW, int W>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
+dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restrict__ tiles,
+                    const int* __restrict__ ntiles, unsigned* __restrict__ ctr) {
+    extern __shared__ unsigned char lds_raw[];
+    __shared__ int s_tile;
+    const int total = ntiles[0];
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = (int)atomicAdd(&ctr[0], 1u);
+        __syncthreads();
+        const int i = __builtin_amdgcn_readfirstlane(s_tile);
+        __syncthreads();
+        if (i >= total) break;
+        const BatchTile tl = tiles[i];
+        const BatchJob& j = jobs[tl.job];
+        dedisp_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
+                                   j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
+                                   j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) {
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
+        }
+    }
+}
+
 // dedisperseSTATS (TIMESERIES:1450-1481), one thread per (dm, t), f32 window.
 __global__ void dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, int nch, int nt,
                                     int ndm, const int* __restrict__ off, float* __restrict__ out,
@@ -392,26 +461,32 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kDefaultTile = 3;
 
 template <typename TS, int I>
-const void* tile_kernel_ptr() {
-    return reinterpret_cast<const void*>(
-        &dedisp_kernel<TS, kTiles[I].sl, kTiles[I].dw, kTiles[I].w>);
+const void* tile_kernel_ptr(bool batch) {
+    constexpr TileCfg c = kTiles[I];
+    return batch ? reinterpret_cast<const void*>(&dedisp_batch_kernel<TS, c.sl, c.dw, c.w>)
+                 : reinterpret_cast<const void*>(&dedisp_kernel<TS, c.sl, c.dw, c.w>);
 }
 template <typename TS>
-const void* tile_kernel(int i) {
+const void* tile_kernel(int i, bool batch) {
     switch (i) {
-        case 0: return tile_kernel_ptr<TS, 0>();
-        case 1: return tile_kernel_ptr<TS, 1>();
-        case 2: return tile_kernel_ptr<TS, 2>();
-        case 3: return tile_kernel_ptr<TS, 3>();
-        case 4: return tile_kernel_ptr<TS, 4>();
-        case 5: return tile_kernel_ptr<TS, 5>();
-        case 6: return tile_kernel_ptr<TS, 6>();
-        case 7: return tile_kernel_ptr<TS, 7>();
-        case 8: return tile_kernel_ptr<TS, 8>();
-        case 9: return tile_kernel_ptr<TS, 9>();
-        case 10: return tile_kernel_ptr<TS, 10>();
-        default: return tile_kernel_ptr<TS, 11>();
+        case 0: return tile_kernel_ptr<TS, 0>(batch);
+        case 1: return tile_kernel_ptr<TS, 1>(batch);
+        case 2: return tile_kernel_ptr<TS, 2>(batch);
+        case 3: return tile_kernel_ptr<TS, 3>(batch);
+        case 4: return tile_kernel_ptr<TS, 4>(batch);
+        case 5: return tile_kernel_ptr<TS, 5>(batch);
+        case 6: return tile_kernel_ptr<TS, 6>(batch);
+        case 7: return tile_kernel_ptr<TS, 7>(batch);
+        case 8: return tile_kernel_ptr<TS, 8>(batch);
+        case 9: return tile_kernel_ptr<TS, 9>(batch);
+        case 10: return tile_kernel_ptr<TS, 10>(batch);
+        default: return tile_kernel_ptr<TS, 11>(batch);
     }
+}
+const void* tile_kernel_for(int storage, int i, bool batch) {
+    if (storage == VFRB_STORE_F32) return tile_kernel<float>(i, batch);
+    if (storage == VFRB_STORE_F16) return tile_kernel<__half>(i, batch);
+    return tile_kernel<uint8_t>(i, batch);
 }
 static_assert(kNumTiles == 12, "tile_kernel() covers 12 configurations");
 
@@ -447,6 +522,32 @@ struct vfrb_search_plan {
     int nblk = 0;
     size_t lds_bytes = 0;
     int loaded_rows = 0;
+    // estimated lane-operations per tile of each DM block (batch ordering)
+    std::vector<int64_t> blk_cost;
+};
+
+struct vfrb_batch {
+    int storage = 0, tile = -1;
+    int max_jobs = 0, max_tiles = 0;
+    int ntiles = -1;  // -1: no jobs set
+    std::vector<vfrb_search_plan*> plans;
+    std::vector<int32_t> nrows;
+    std::vector<float*> outs;
+    std::vector<int64_t> ld_outs;
+    // pinned staging of the work list and its device copy
+    BatchJob* h_jobs = nullptr;
+    BatchTile* h_tiles = nullptr;
+    int* h_ntiles = nullptr;
+    BatchJob* d_jobs = nullptr;
+    BatchTile* d_tiles = nullptr;
+    int* d_ntiles = nullptr;
+    unsigned* d_ctr = nullptr;
+    hipEvent_t staged = nullptr;  // recorded after the staging upload
+    bool staged_pending = false;
+    int grid = 0;
+    size_t lds_bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -479,6 +580,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     std::vector<Chunk> chunks;
     std::vector<uint32_t> roffs;
     std::vector<int2> blk(nblk);
+    std::vector<int64_t> cost(nblk, 0);
     // A staged channel takes 4*stride bytes in every storage type: one f32
     // copy, two shifted f16 copies or four shifted u8 copies.
     auto stride_of = [&](int s) { return (T + s + 3) / 4 * 4; };
@@ -519,6 +621,8 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
                 }
             }
             max_data = std::max(max_data, (size_t)count * 4 * stride);
+            // accumulate + staging (one load and EPD LDS writes per element)
+            cost[b] += (int64_t)count * ((int64_t)DMB * T + 2LL * stride * epd);
             c += count;
         }
         blk[b].y = (int)chunks.size() - blk[b].x;
@@ -538,14 +642,13 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     }
     p->nblk = nblk;
     p->tile = i;
+    p->blk_cost = std::move(cost);
     p->lds_bytes = (size_t)kChunkMax * DMB * 4 + max_data;
     return VFRB_OK;
 }
 
 const void* dedisp_kernel_for(const vfrb_search_plan* p) {
-    if (p->storage == VFRB_STORE_F32) return tile_kernel<float>(p->tile);
-    if (p->storage == VFRB_STORE_F16) return tile_kernel<__half>(p->tile);
-    return tile_kernel<uint8_t>(p->tile);
+    return tile_kernel_for(p->storage, p->tile, false);
 }
 
 void destroy(vfrb_search_plan* p) {
@@ -784,5 +887,277 @@ vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* p, int32_t nrows, flo
     VFRB_LAUNCH_CHECK("dedisp_stats_kernel");
     return VFRB_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched dedispersion (host side)
+// ---------------------------------------------------------------------------
+namespace {
+
+void drop_graph(vfrb_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// LDS of a batch launch: the largest any plan can need with tile i, so one
+// captured launch serves every job set.
+vfrb_status batch_configure(vfrb_batch* b, int i) {
+    const TileCfg& tc = kTiles[i];
+    const size_t lds = (size_t)kChunkMax * tc.w * tc.dw * 4 + kDataBudget;
+    const void* kern = tile_kernel_for(b->storage, i, true);
+    VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int dev = 0, per_cu = 0, cus = 0;
+    VFRB_HIP(hipGetDevice(&dev));
+    VFRB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    VFRB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * tc.w, lds));
+    VFRB_REQUIRE(per_cu > 0, VFRB_ERR_UNSUPPORTED, "tile %s does not fit a compute unit", tc.name);
+    drop_graph(b);
+    b->tile = i;
+    b->lds_bytes = lds;
+    b->grid = per_cu * cus;
+    return VFRB_OK;
+}
+
+// Builds the work list of the stored jobs for the batch's tile configuration
+// and uploads it on stream s.  Work items are ordered by the estimated cost
+// of their DM block, largest first, so the persistent grid drains evenly.
+vfrb_status batch_build(vfrb_batch* b, hipStream_t s) {
+    const int njobs = (int)b->plans.size();
+    if (b->staged_pending) {
+        VFRB_HIP(hipEventSynchronize(b->staged));
+        b->staged_pending = false;
+    }
+    const TileCfg& tc = kTiles[b->tile];
+    const int T = 64 * tc.sl;
+    struct Blk {
+        int64_t cost;
+        int job, blk;
+    };
+    std::vector<Blk> blks;
+    int64_t total = 0;
+    for (int j = 0; j < njobs; j++) {
+        vfrb_search_plan* p = b->plans[j];
+        if (p->tile != b->tile) {
+            const vfrb_status st = build_tiles(p, b->tile);
+            if (st != VFRB_OK) return st;
+        }
+        const int nrows = b->nrows[j];
+        const int nt = nrows - p->max_delay;
+        BatchJob& jb = b->h_jobs[j];
+        jb.win = p->d_win;
+        jb.ld = p->ld;
+        jb.nch = p->nch;
+        jb.nrows = nrows;
+        jb.nt = nt;
+        jb.ndm = p->ndm;
+        jb.roffs = p->d_roffs;
+        jb.dmin = p->d_dmin;
+        jb.chunks = p->d_chunks;
+        jb.blk_chunks = p->d_blk_chunks;
+        jb.chan_scale = p->d_u8_scale;
+        jb.f16_exp = p->d_f16_exp;
+        jb.offsum = p->d_u8_offsum;
+        jb.out = b->outs[j];
+        jb.ld_out = b->ld_outs[j];
+        for (int k = 0; k < p->nblk; k++) blks.push_back(Blk{p->blk_cost[k], j, k});
+        total += (int64_t)p->nblk * vfrb::ceil_div(nt, T);
+    }
+    VFRB_REQUIRE(total <= b->max_tiles, VFRB_ERR_CAPACITY,
+                 "%lld work items exceed the batch capacity %d", (long long)total, b->max_tiles);
+    std::stable_sort(blks.begin(), blks.end(), [](const Blk& x, const Blk& y) { return x.cost > y.cost; });
+    int n = 0;
+    for (const Blk& k : blks) {
+        const int ntt = (int)vfrb::ceil_div(b->h_jobs[k.job].nt, T);
+        for (int x = 0; x < ntt; x++) b->h_tiles[n++] = BatchTile{k.job, k.blk, x, 0};
+    }
+    *b->h_ntiles = n;
+    VFRB_HIP(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(BatchJob) * (size_t)std::max(njobs, 1),
+                            hipMemcpyHostToDevice, s));
+    VFRB_HIP(hipMemcpyAsync(b->d_tiles, b->h_tiles, sizeof(BatchTile) * (size_t)std::max(n, 1),
+                            hipMemcpyHostToDevice, s));
+    VFRB_HIP(hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(int), hipMemcpyHostToDevice, s));
+    VFRB_HIP(hipEventRecord(b->staged, s));
+    b->staged_pending = true;
+    b->ntiles = n;
+    return VFRB_OK;
+}
+
+vfrb_status batch_launch(vfrb_batch* b, hipStream_t s) {
+    const TileCfg& tc = kTiles[b->tile];
+    const void* kern = tile_kernel_for(b->storage, b->tile, true);
+    void* args[] = {(void*)&b->d_jobs, (void*)&b->d_tiles, (void*)&b->d_ntiles, (void*)&b->d_ctr};
+    VFRB_HIP(hipLaunchKernel(kern, dim3((unsigned)b->grid), dim3(64 * tc.w), args, b->lds_bytes, s));
+    return VFRB_OK;
+}
+
+void batch_free(vfrb_batch* b) {
+    drop_graph(b);
+    if (b->staged) {
+        if (b->staged_pending) (void)hipEventSynchronize(b->staged);
+        (void)hipEventDestroy(b->staged);
+    }
+    (void)hipHostFree(b->h_jobs);
+    (void)hipHostFree(b->h_tiles);
+    (void)hipHostFree(b->h_ntiles);
+    (void)hipFree(b->d_jobs);
+    (void)hipFree(b->d_tiles);
+    (void)hipFree(b->d_ntiles);
+    (void)hipFree(b->d_ctr);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+vfrb_status vfrb_batch_create(vfrb_batch** batch, int32_t storage, int32_t tile_config,
+                              int32_t max_jobs, int32_t max_tiles) {
+    VFRB_REQUIRE(batch, VFRB_ERR_INVALID, "NULL argument");
+    *batch = nullptr;
+    VFRB_REQUIRE(storage >= VFRB_STORE_F32 && storage <= VFRB_STORE_U8, VFRB_ERR_INVALID,
+                 "storage=%d", storage);
+    VFRB_REQUIRE(tile_config >= -1 && tile_config < kNumTiles, VFRB_ERR_INVALID,
+                 "tile_config=%d", tile_config);
+    VFRB_REQUIRE(max_jobs > 0 && max_tiles > 0, VFRB_ERR_INVALID, "max_jobs=%d max_tiles=%d",
+                 max_jobs, max_tiles);
+    vfrb_batch* b = new vfrb_batch();
+    b->storage = storage;
+    b->max_jobs = max_jobs;
+    b->max_tiles = max_tiles;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&b->h_jobs), sizeof(BatchJob) * max_jobs);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_tiles), sizeof(BatchTile) * (size_t)max_tiles);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&b->h_ntiles), sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->d_jobs, sizeof(BatchJob) * max_jobs);
+    if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(BatchTile) * (size_t)max_tiles);
+    if (e == hipSuccess) e = hipMalloc(&b->d_ntiles, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->d_ctr, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(b->d_ctr, 0, 2 * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(b->d_ntiles, 0, sizeof(int));
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        const vfrb_status st = vfrb::hip_fail(e, "vfrb_batch_create", __FILE__, __LINE__);
+        batch_free(b);
+        return st;
+    }
+    const vfrb_status st = batch_configure(b, tile_config >= 0 ? tile_config : kDefaultTile);
+    if (st != VFRB_OK) {
+        batch_free(b);
+        return st;
+    }
+    *batch = b;
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_batch_destroy(vfrb_batch* batch) {
+    if (batch) batch_free(batch);
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_batch_set(vfrb_batch* b, int32_t njobs, vfrb_search_plan* const* plans,
+                           const int32_t* nrows, float* const* outs, const int64_t* ld_outs,
+                           int32_t* ntiles, void* stream) {
+    VFRB_REQUIRE(b && njobs >= 0, VFRB_ERR_INVALID, "bad batch arguments");
+    VFRB_REQUIRE(njobs == 0 || (plans && nrows && outs && ld_outs), VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(njobs <= b->max_jobs, VFRB_ERR_CAPACITY, "%d jobs exceed the batch capacity %d",
+                 njobs, b->max_jobs);
+    for (int j = 0; j < njobs; j++) {
+        const vfrb_search_plan* p = plans[j];
+        VFRB_REQUIRE(p && outs[j], VFRB_ERR_INVALID, "job %d: NULL plan or output", j);
+        VFRB_REQUIRE(p->storage == b->storage, VFRB_ERR_INVALID,
+                     "job %d: plan storage %d differs from the batch's %d", j, p->storage, b->storage);
+        const int nt = nrows[j] - p->max_delay;
+        VFRB_REQUIRE(nrows[j] <= p->loaded_rows && nt > 0, VFRB_ERR_INVALID,
+                     "job %d: %d rows requested, %d loaded, max_delay %d", j, nrows[j],
+                     p->loaded_rows, p->max_delay);
+        VFRB_REQUIRE(ld_outs[j] >= nt, VFRB_ERR_INVALID, "job %d: ld_out=%lld < %d", j,
+                     (long long)ld_outs[j], nt);
+    }
+    b->plans.assign(plans, plans + njobs);
+    b->nrows.assign(nrows, nrows + njobs);
+    b->outs.assign(outs, outs + njobs);
+    b->ld_outs.assign(ld_outs, ld_outs + njobs);
+    const vfrb_status st = batch_build(b, vfrb::as_stream(stream));
+    if (st != VFRB_OK) {
+        b->ntiles = -1;
+        return st;
+    }
+    if (ntiles) *ntiles = b->ntiles;
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_batch_dedisperse(vfrb_batch* b, void* stream) {
+    VFRB_REQUIRE(b && b->ntiles >= 0, VFRB_ERR_INVALID, "no jobs set");
+    return batch_launch(b, vfrb::as_stream(stream));
+}
+
+vfrb_status vfrb_batch_graph_launch(vfrb_batch* b, void* stream) {
+    VFRB_REQUIRE(b && b->ntiles >= 0, VFRB_ERR_INVALID, "no jobs set");
+    VFRB_REQUIRE(stream, VFRB_ERR_INVALID, "graph capture needs a created stream");
+    hipStream_t s = vfrb::as_stream(stream);
+    if (!b->exec) {
+        VFRB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const vfrb_status st = batch_launch(b, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (st != VFRB_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        if (e != hipSuccess) return vfrb::hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        b->graph = g;
+        const hipError_t e2 = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+            drop_graph(b);
+            return vfrb::hip_fail(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+    }
+    VFRB_HIP(hipGraphLaunch(b->exec, s));
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_batch_autotune(vfrb_batch* b, int32_t iters, float* ms, void* stream) {
+    VFRB_REQUIRE(b && b->ntiles >= 0 && iters > 0, VFRB_ERR_INVALID, "bad autotune arguments");
+    hipStream_t s = vfrb::as_stream(stream);
+    hipEvent_t e0, e1;
+    VFRB_HIP(hipEventCreate(&e0));
+    VFRB_HIP(hipEventCreate(&e1));
+    const int keep = b->tile;
+    int best = -1;
+    float best_ms = 0.0f;
+    vfrb_status st = VFRB_OK;
+    for (int i = 0; i < kNumTiles; i++) {
+        if (ms) ms[i] = -1.0f;
+        if (batch_configure(b, i) != VFRB_OK || batch_build(b, s) != VFRB_OK) continue;
+        st = batch_launch(b, s);  // warm-up
+        if (st != VFRB_OK) break;
+        if (hipEventRecord(e0, s) != hipSuccess) { st = VFRB_ERR_HIP; break; }
+        for (int k = 0; k < iters && st == VFRB_OK; k++) st = batch_launch(b, s);
+        if (st != VFRB_OK) break;
+        float t = 0.0f;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+            st = VFRB_ERR_HIP;
+            break;
+        }
+        t /= iters;
+        if (ms) ms[i] = t;
+        if (best < 0 || t < best_ms) {
+            best = i;
+            best_ms = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    vfrb_status st2 = batch_configure(b, best >= 0 ? best : keep);
+    if (st2 == VFRB_OK) st2 = batch_build(b, s);
+    if (st != VFRB_OK) return st;
+    return st2;
+}
+
+int32_t vfrb_batch_get_tile(const vfrb_batch* b) { return b ? b->tile : -1; }
+int32_t vfrb_batch_grid(const vfrb_batch* b) { return b ? b->grid : -1; }
 
 }  // extern "C"

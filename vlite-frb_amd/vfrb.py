@@ -19,6 +19,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libvfrb.so")
 STORE_F32, STORE_F16, STORE_U8 = 0, 1, 2
 ACC_F64, ACC_F32C = 0, 1
 STORAGE_NAMES = {"f32": STORE_F32, "f16": STORE_F16, "u8": STORE_U8}
+(OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_NOMEM, ERR_HIP, ERR_NODEVICE, ERR_IO, ERR_FORMAT, ERR_EOF,
+ ERR_CAPACITY) = range(10)
 
 
 class VfrbError(RuntimeError):
@@ -117,6 +119,14 @@ _SIGS = {
     "vfrb_search_set_tile": (i32, [vp, i32]),
     "vfrb_search_get_tile": (i32, [vp]),
     "vfrb_search_autotune": (i32, [vp, i32, vp, i64, i32, P(f32), vp]),
+    "vfrb_batch_create": (i32, [P(vp), i32, i32, i32, i32]),
+    "vfrb_batch_destroy": (i32, [vp]),
+    "vfrb_batch_set": (i32, [vp, i32, P(vp), P(i32), P(vp), P(i64), P(i32), vp]),
+    "vfrb_batch_dedisperse": (i32, [vp, vp]),
+    "vfrb_batch_graph_launch": (i32, [vp, vp]),
+    "vfrb_batch_autotune": (i32, [vp, i32, P(f32), vp]),
+    "vfrb_batch_get_tile": (i32, [vp]),
+    "vfrb_batch_grid": (i32, [vp]),
     "vfrb_dm_stats": (i32, [vp, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "vfrb_candidates": (i32, [vp, i64, i32, i32, vp, vp, f32, vp, vp, vp, i32, vp, vp]),
     "vfrb_vdif_decode": (i32, [P(C.c_uint8), P(VdifHeader)]),
@@ -466,6 +476,57 @@ class SearchPlan:
         try:
             if self.h:
                 lib.vfrb_search_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Batch:
+    """Persistent batched dedispersion over many search plans
+    (vfrb_batch_*): set() the jobs, then dedisperse() eagerly or
+    graph_launch() a captured hipGraph replay."""
+
+    def __init__(self, storage=STORE_F32, tile=-1, max_jobs=64, max_tiles=1 << 20):
+        self.h = vp()
+        call("vfrb_batch_create", C.byref(self.h), storage, tile, max_jobs, max_tiles)
+        self._keep = ()
+
+    def set(self, jobs, stream=None):
+        """jobs: sequence of (SearchPlan, nrows, out DeviceArray, ld_out)."""
+        n = len(jobs)
+        plans = (vp * max(n, 1))(*[j[0].h for j in jobs])
+        nrows = np.array([j[1] for j in jobs] or [0], dtype=np.int32)
+        outs = (vp * max(n, 1))(*[j[2].ptr for j in jobs])
+        lds = np.array([j[3] for j in jobs] or [0], dtype=np.int64)
+        nt = i32(0)
+        call("vfrb_batch_set", self.h, n, plans, _ptr(nrows, i32), outs, _ptr(lds, i64),
+             C.byref(nt), _s(stream))
+        self._keep = tuple(jobs)  # plans and outputs must outlive the work list
+        return nt.value
+
+    def dedisperse(self, stream=None):
+        call("vfrb_batch_dedisperse", self.h, _s(stream))
+
+    def graph_launch(self, stream):
+        call("vfrb_batch_graph_launch", self.h, _s(stream))
+
+    def autotune(self, iters=2, stream=None):
+        n = lib.vfrb_search_tile_count()
+        ms = np.zeros(n, np.float32)
+        call("vfrb_batch_autotune", self.h, iters, _ptr(ms, f32), _s(stream))
+        return dict(zip(tile_names(), ms.tolist()))
+
+    @property
+    def tile(self):
+        return lib.vfrb_batch_get_tile(self.h)
+
+    @property
+    def grid(self):
+        return lib.vfrb_batch_grid(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_batch_destroy(self.h)
         except Exception:
             pass
 
