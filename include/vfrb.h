@@ -326,6 +326,118 @@ int32_t     vfrb_batch_get_tile(const vfrb_batch* batch);
 int32_t     vfrb_batch_grid(const vfrb_batch* batch);
 
 /* ------------------------------------------------------------------------ */
+/* Segment server (no reference counterpart: the reference processes one    */
+/* VDIF stream, window by window, in a single host loop, PIPELINE:560-760)   */
+/* ------------------------------------------------------------------------ */
+/*
+ * A request loop over many antenna streams.  Each stream keeps on the device
+ * its raw filterbank ring (the max-delay overlap the next window needs, the
+ * reference's tail save/restore of PIPELINE:653,696), the partial row of
+ * payload bytes left over by the previous segment, its search plan (delay
+ * table, window, storage scales) and its output buffers.  A segment of any
+ * byte length is filterbanked onto the end of its stream; the window is the
+ * last min(rows so far, max_delay + new rows) rows, bandpassed
+ * (subtractbpCAL), dedispersed, and reduced to per-DM statistics (calcstats)
+ * and a candidate list.
+ *
+ * batched = 1: continuous batching.  A scheduler thread admits waiting
+ * segments (arrival order, one per stream at a time) while the device input
+ * arena has room, up to max_batch per iteration, and runs an iteration on
+ * whichever of two HIP streams is free, so a new iteration overlaps the one
+ * in flight.  The dedispersion of all segments of an iteration is one
+ * persistent vfrb_batch launch replayed from a captured hipGraph.
+ * batched = 0: the plain path: one segment at a time, in arrival order, with
+ * one dedispersion launch per segment.
+ * Results are the same whichever segments share an iteration.
+ */
+typedef struct vfrb_server_desc {
+    int32_t twonchan;      /* FFT length                                      */
+    int32_t nave;          /* spectra per output row                          */
+    int32_t nchans;        /* channels kept (NCHANOUT)                        */
+    int32_t nskip;         /* first kept R2C bin (NSKIP)                      */
+    double  fch1;          /* MHz, centre of channel 0                        */
+    double  chanbw;        /* MHz                                             */
+    double  tsamp;         /* s per row                                       */
+    int32_t storage;       /* VFRB_STORE_* of the dedispersion window         */
+    int32_t acc;           /* VFRB_ACC_* of the per-DM statistics             */
+    float   threshold;     /* candidate SNR threshold (strict >)              */
+    int32_t max_cand;      /* candidates kept per segment                     */
+    int32_t max_streams;
+    int64_t max_segment_bytes;  /* per polarisation                           */
+    int64_t arena_bytes;   /* device input arena: admission limit             */
+    int32_t max_batch;     /* segments per iteration (batched mode)           */
+    int32_t batched;       /* 1 = continuous batching, 0 = plain FIFO         */
+    int32_t tile_config;   /* dedispersion tile, -1 = default                 */
+} vfrb_server_desc;
+
+/* Caller-owned host buffers that receive one segment's results; any may be
+ * NULL.  They must stay valid until the segment's result is polled. */
+typedef struct vfrb_server_output {
+    float*   peak;      /* [ndm] peak SNR per DM (calcstats, PIPELINE:1337-1370) */
+    int32_t* peak_t;    /* [ndm] absolute output sample of that peak          */
+    float*   ave;       /* [ndm]                                              */
+    float*   sigma;     /* [ndm]                                              */
+    int32_t* cand_dm;   /* [max_cand] candidates sorted by (dm, t)            */
+    int32_t* cand_t;    /* [max_cand] absolute output sample                  */
+    float*   cand_snr;  /* [max_cand]                                         */
+} vfrb_server_output;
+
+typedef struct vfrb_server_result {
+    int64_t tag;
+    int32_t stream_id;
+    int32_t status;      /* VFRB_OK or this segment's error                  */
+    int64_t rows;        /* filterbank rows the segment completed            */
+    int64_t sample0;     /* absolute output sample of the window's t = 0     */
+    int32_t nt;          /* samples searched; 0 while the stream holds no
+                            more than max_delay + 1 rows                     */
+    int32_t ncand;       /* candidates above threshold (may exceed max_cand) */
+    int32_t best_dm;     /* DM index of the largest peak SNR, -1 if nt == 0  */
+    int32_t best_t;      /* absolute sample of that peak                     */
+    float   best_snr;
+    int32_t batch_size;  /* segments in the same iteration                   */
+    double  arrival_s;   /* server clock (s since create) at submit          */
+    double  start_s;     /* iteration issued to the device                   */
+    double  done_s;      /* results on the host                              */
+} vfrb_server_result;
+
+typedef struct vfrb_server_stats {
+    int64_t submitted, completed, iterations, batched_segments;
+    int64_t arena_peak_bytes;
+    double  device_busy_s;  /* sum of iteration device times                 */
+} vfrb_server_stats;
+
+typedef struct vfrb_server vfrb_server;
+
+vfrb_status vfrb_server_create(vfrb_server** server, const vfrb_server_desc* desc);
+/* Waits for every submitted segment, then frees everything. */
+vfrb_status vfrb_server_destroy(vfrb_server* server);
+/* Adds a stream with its own DM list (host floats, ascending, last = largest).
+ * Fails with VFRB_ERR_NOMEM when free device memory cannot hold its state. */
+vfrb_status vfrb_server_open_stream(vfrb_server* server, const float* dms, int32_t ndm,
+                                    int32_t* stream_id);
+int32_t     vfrb_server_max_delay(const vfrb_server* server, int32_t stream_id);
+/*
+ * Queues nbytes of payload per polarisation (host memory, valid until the
+ * result is polled) as the next segment of stream_id; returns at once.
+ * VFRB_ERR_CAPACITY when nbytes exceeds max_segment_bytes.
+ */
+vfrb_status vfrb_server_submit(vfrb_server* server, int32_t stream_id, const uint8_t* pol0,
+                               const uint8_t* pol1, int64_t nbytes, int64_t tag,
+                               const vfrb_server_output* out);
+/*
+ * Moves up to max_results finished results (completion order) into results;
+ * waits up to wait_ms (< 0: until one is ready or nothing is outstanding) when
+ * none is ready.  *n receives the count.
+ */
+vfrb_status vfrb_server_poll(vfrb_server* server, vfrb_server_result* results,
+                             int32_t max_results, int32_t wait_ms, int32_t* n);
+/* Blocks until every submitted segment has finished. */
+vfrb_status vfrb_server_drain(vfrb_server* server);
+vfrb_status vfrb_server_get_stats(const vfrb_server* server, vfrb_server_stats* stats);
+/* Server clock: seconds since vfrb_server_create (steady clock). */
+double      vfrb_server_now(const vfrb_server* server);
+
+/* ------------------------------------------------------------------------ */
 /* Per-DM statistics and candidates                                         */
 /* ------------------------------------------------------------------------ */
 /*

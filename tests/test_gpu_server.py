@@ -1,0 +1,169 @@
+"""GPU: the segment server (vfrb_server_*) against the oracle replay of its
+window rules, and the same stream alone vs in a mix of streams."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, NAVE = 2048, 8
+DMS = (np.arange(32) * 0.1).astype(np.float32)
+# uneven segment lengths in rows (fractions leave partial rows for the carry)
+SEG_ROWS = [350.5, 200.25, 611.75, 90.1, 480.0, 260.6]
+
+
+def _shape(vfrb):
+    return vfrb.params_derive(N, NAVE, 1, 1.0, fullband=True)
+
+
+def _stream_bytes(vfrb, p, seed, pulse_t0):
+    nbytes = [int(r * N * NAVE) for r in SEG_ROWS]
+    total = sum(nbytes)
+    nblk = -(-total // N)
+    amp = vfrb.amp_for_snr(30, NAVE, p.nchanout)
+    spec = vfrb.synth_spec(N, seed=seed, pulses=[(2.0, pulse_t0, 0.002, amp)])
+    p0 = vfrb.synth_payload(spec, 0, 0, nblk)[:total]
+    p1 = vfrb.synth_payload(spec, 1, 0, nblk)[:total]
+    cuts = np.concatenate([[0], np.cumsum(nbytes)])
+    return p0, p1, cuts
+
+
+def _server(vfrb, p, storage, batched, **kw):
+    return vfrb.Server(N, NAVE, p.nchanout, p.nskip, p.fch1, p.chanbw, p.tchan, storage=storage,
+                       acc=kw.pop("acc", vfrb.ACC_F64), threshold=7.0, max_cand=8192,
+                       max_streams=kw.pop("max_streams", 8), max_segment_bytes=16 << 20,
+                       arena_bytes=kw.pop("arena_bytes", 1 << 28), max_batch=8, batched=batched, **kw)
+
+
+def _run(srv, streams):
+    """streams: list of (sid, p0, p1, cuts, tag_base); submits segment k of
+    every stream before segment k+1 of any, then collects all results."""
+    nseg = len(SEG_ROWS)
+    for k in range(nseg):
+        for sid, p0, p1, cuts, base in streams:
+            a, b = cuts[k], cuts[k + 1]
+            srv.submit(sid, p0[a:b], p1[a:b], base + k)
+    srv.drain()
+    res = {}
+    while True:
+        got = srv.poll(wait_ms=0)
+        if not got:
+            break
+        for r in got:
+            res[r["tag"]] = r
+    assert len(res) == nseg * len(streams)
+    return res
+
+
+def test_server_windows_match_oracle_replay(gpu, orc):
+    vfrb = gpu
+    p = _shape(vfrb)
+    print("[case] server f32 plain vs oracle replay", flush=True)
+    p0, p1, cuts = _stream_bytes(vfrb, p, seed=3, pulse_t0=0.15)
+    srv = _server(vfrb, p, vfrb.STORE_F32, batched=False)
+    sid = srv.open_stream(DMS)
+    maxd = srv.max_delay(sid)
+    res = _run(srv, [(sid, p0, p1, cuts, 0)])
+    srv.close()
+    rb = N * NAVE
+    total_rows = int(cuts[-1] // rb)
+    fb = orc.filterbank_f32(p0, p1, total_rows, N, NAVE, p.nskip, p.nchanout)
+    off, maxd_o = orc.delay_table(DMS, p.nchanout, p.fch1, p.chanbw, p.tchan)
+    assert maxd == maxd_o
+    prev = 0
+    searched = 0
+    for k in range(len(SEG_ROWS)):
+        r = res[k]
+        tot = int(cuts[k + 1] // rb)
+        R = tot - prev
+        prev = tot
+        W = min(tot, maxd + R)
+        nt = W - maxd
+        assert r["status"] == 0 and r["rows"] == R
+        if nt < 2:
+            assert r["nt"] == 0 and r["best_dm"] == -1
+            continue
+        searched += 1
+        assert r["nt"] == nt and r["sample0"] == tot - W
+        out = orc.dedisperse_f32(orc.bandpass_f32(fb[tot - W: tot]), off, nt)
+        _, ave, sig, peak, pt = orc.dm_stats(out)
+        assert np.max(np.abs(r["peak"] - peak) / np.maximum(peak, 1.0)) < 2e-3
+        assert np.max(np.abs(r["ave"] - ave) / sig) < 1e-3
+        assert np.max(np.abs(r["sigma"] / sig - 1)) < 1e-3
+        b = int(np.argmax(peak))
+        if peak[b] > 10:  # the pulse is inside this window
+            assert r["best_dm"] == b and r["best_t"] == pt[b] + tot - W
+    assert searched >= 4
+
+
+def test_server_result_is_the_same_alone_and_in_the_mix(gpu):
+    vfrb = gpu
+    p = _shape(vfrb)
+    a = _stream_bytes(vfrb, p, seed=5, pulse_t0=0.12)
+    others = [_stream_bytes(vfrb, p, seed=s, pulse_t0=0.05 * s) for s in (6, 7, 8)]
+    print("[case] server f16: stream alone (batched)", flush=True)
+    s1 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    sid = s1.open_stream(DMS)
+    alone = _run(s1, [(sid, *a, 0)])
+    s1.close()
+    print("[case] server f16: stream alone (plain)", flush=True)
+    s2 = _server(vfrb, p, vfrb.STORE_F16, batched=False)
+    sid = s2.open_stream(DMS)
+    plain = _run(s2, [(sid, *a, 0)])
+    s2.close()
+    print("[case] server f16: stream in a mix of four", flush=True)
+    s3 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    ids = [s3.open_stream(DMS) for _ in range(4)]
+    # stream A is the second one opened; the others get their own DM lists
+    streams = [(ids[0], *others[0], 100), (ids[1], *a, 0), (ids[2], *others[1], 200),
+               (ids[3], *others[2], 300)]
+    mix = _run(s3, streams)
+    stats = s3.stats()
+    s3.close()
+    assert stats["completed"] == 4 * len(SEG_ROWS)
+    assert max(r["batch_size"] for r in mix.values()) > 1, "no iteration batched two segments"
+    keys = ("status", "rows", "sample0", "nt", "ncand", "best_dm", "best_t", "best_snr")
+    arrays = ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr")
+    for k in range(len(SEG_ROWS)):
+        for other in (plain[k], mix[k]):
+            for key in keys:
+                assert alone[k][key] == other[key], (k, key)
+            if alone[k]["nt"]:
+                for key in arrays:
+                    assert np.array_equal(alone[k][key], other[key]), (k, key)
+    assert any(alone[k]["ncand"] > 0 for k in alone), "the pulse gave no candidate"
+    for r in mix.values():
+        assert r["arrival_s"] <= r["start_s"] <= r["done_s"]
+
+
+def test_server_admission_and_errors(gpu):
+    vfrb = gpu
+    p = _shape(vfrb)
+    rb = N * NAVE
+    print("[case] server admission limits", flush=True)
+    # an arena that holds one 300-row segment (both pols) at a time
+    arena = 2 * (300 * rb + rb + 4096)
+    srv = _server(vfrb, p, vfrb.STORE_F16, batched=True, arena_bytes=arena, max_streams=2)
+    s0, s1 = srv.open_stream(DMS), srv.open_stream(DMS)
+    with pytest.raises(vfrb.VfrbError) as e:
+        srv.open_stream(DMS)
+    assert e.value.status == vfrb.ERR_CAPACITY
+    seg = np.full(300 * rb, 128, np.uint8)
+    with pytest.raises(vfrb.VfrbError) as e:
+        srv.submit(s0, np.zeros(17 << 20, np.uint8), np.zeros(17 << 20, np.uint8), 99)
+    assert e.value.status == vfrb.ERR_CAPACITY
+    with pytest.raises(vfrb.VfrbError):
+        srv.submit(7, seg, seg, 98)
+    for k in range(4):
+        srv.submit(s0 if k % 2 == 0 else s1, seg, seg, k)
+    srv.drain()
+    got = []
+    while True:
+        r = srv.poll(wait_ms=0)
+        if not r:
+            break
+        got += r
+    st = srv.stats()
+    srv.close()
+    assert sorted(r["tag"] for r in got) == [0, 1, 2, 3]
+    assert all(r["status"] == 0 and r["batch_size"] == 1 for r in got)
+    assert st["arena_peak_bytes"] <= arena and st["iterations"] == 4

@@ -71,6 +71,29 @@ class SynthSpec(C.Structure):
                 ("tone_bin", f64 * MAXT), ("tone_amp", f64 * MAXT), ("nthreads", i32)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("twonchan", i32), ("nave", i32), ("nchans", i32), ("nskip", i32), ("fch1", f64),
+                ("chanbw", f64), ("tsamp", f64), ("storage", i32), ("acc", i32), ("threshold", f32),
+                ("max_cand", i32), ("max_streams", i32), ("max_segment_bytes", i64),
+                ("arena_bytes", i64), ("max_batch", i32), ("batched", i32), ("tile_config", i32)]
+
+
+class ServerOutput(C.Structure):
+    _fields_ = [("peak", P(f32)), ("peak_t", P(i32)), ("ave", P(f32)), ("sigma", P(f32)),
+                ("cand_dm", P(i32)), ("cand_t", P(i32)), ("cand_snr", P(f32))]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("tag", i64), ("stream_id", i32), ("status", i32), ("rows", i64), ("sample0", i64),
+                ("nt", i32), ("ncand", i32), ("best_dm", i32), ("best_t", i32), ("best_snr", f32),
+                ("batch_size", i32), ("arrival_s", f64), ("start_s", f64), ("done_s", f64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", i64), ("completed", i64), ("iterations", i64),
+                ("batched_segments", i64), ("arena_peak_bytes", i64), ("device_busy_s", f64)]
+
+
 LOG_FN = C.CFUNCTYPE(None, C.c_char_p, vp)
 
 _SIGS = {
@@ -127,6 +150,15 @@ _SIGS = {
     "vfrb_batch_autotune": (i32, [vp, i32, P(f32), vp]),
     "vfrb_batch_get_tile": (i32, [vp]),
     "vfrb_batch_grid": (i32, [vp]),
+    "vfrb_server_create": (i32, [P(vp), P(ServerDesc)]),
+    "vfrb_server_destroy": (i32, [vp]),
+    "vfrb_server_open_stream": (i32, [vp, P(f32), i32, P(i32)]),
+    "vfrb_server_max_delay": (i32, [vp, i32]),
+    "vfrb_server_submit": (i32, [vp, i32, vp, vp, i64, i64, P(ServerOutput)]),
+    "vfrb_server_poll": (i32, [vp, P(ServerResult), i32, i32, P(i32)]),
+    "vfrb_server_drain": (i32, [vp]),
+    "vfrb_server_get_stats": (i32, [vp, P(ServerStats)]),
+    "vfrb_server_now": (f64, [vp]),
     "vfrb_dm_stats": (i32, [vp, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "vfrb_candidates": (i32, [vp, i64, i32, i32, vp, vp, f32, vp, vp, vp, i32, vp, vp]),
     "vfrb_vdif_decode": (i32, [P(C.c_uint8), P(VdifHeader)]),
@@ -527,6 +559,115 @@ class Batch:
         try:
             if self.h:
                 lib.vfrb_batch_destroy(self.h)
+        except Exception:
+            pass
+
+
+class PinnedBytes:
+    """Pinned host buffer (vfrb_host_alloc) exposed as a numpy uint8 array."""
+
+    def __init__(self, nbytes):
+        self.ptr = vp()
+        call("vfrb_host_alloc", C.byref(self.ptr), max(int(nbytes), 1))
+        self.array = np.ctypeslib.as_array(C.cast(self.ptr, P(C.c_uint8)), shape=(int(nbytes),))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib.vfrb_host_free(self.ptr)
+        except Exception:
+            pass
+
+
+class Server:
+    """Segment server (vfrb_server_*).  submit() returns at once; poll()
+    hands back finished segments as dicts with their per-DM results and
+    candidates (numpy arrays owned by the result)."""
+
+    def __init__(self, twonchan, nave, nchans, nskip, fch1, chanbw, tsamp, storage=STORE_F16,
+                 acc=ACC_F32C, threshold=7.0, max_cand=4096, max_streams=16,
+                 max_segment_bytes=1 << 28, arena_bytes=1 << 32, max_batch=16, batched=True,
+                 tile=-1):
+        d = ServerDesc(twonchan, nave, nchans, nskip, fch1, chanbw, tsamp, storage, acc, threshold,
+                       max_cand, max_streams, max_segment_bytes, arena_bytes, max_batch,
+                       int(bool(batched)), tile)
+        self.max_cand = max_cand
+        self.h = vp()
+        call("vfrb_server_create", C.byref(self.h), C.byref(d))
+        self._ndm = {}
+        self._live = {}  # tag -> (payloads, outputs) kept until polled
+
+    def open_stream(self, dms):
+        dms = np.ascontiguousarray(dms, dtype=np.float32)
+        sid = i32(0)
+        call("vfrb_server_open_stream", self.h, _ptr(dms, f32), len(dms), C.byref(sid))
+        self._ndm[sid.value] = len(dms)
+        return sid.value
+
+    def max_delay(self, sid):
+        return lib.vfrb_server_max_delay(self.h, sid)
+
+    def submit(self, sid, pol0, pol1, tag, outputs=True):
+        """pol0/pol1: contiguous uint8 arrays of the same length (views of
+        pinned_bytes() memory copy fastest); referenced until polled."""
+        assert pol0.dtype == np.uint8 and pol1.dtype == np.uint8 and len(pol0) == len(pol1)
+        assert pol0.flags.c_contiguous and pol1.flags.c_contiguous
+        k0, k1, n = pol0, pol1, len(pol0)
+        a0, a1 = pol0.ctypes.data, pol1.ctypes.data
+        ndm = self._ndm.get(sid, 1)  # unknown ids are rejected by the library
+        outs = None
+        o = None
+        if outputs:
+            outs = {"peak": np.zeros(ndm, np.float32), "peak_t": np.zeros(ndm, np.int32),
+                    "ave": np.zeros(ndm, np.float32), "sigma": np.zeros(ndm, np.float32),
+                    "cand_dm": np.zeros(self.max_cand, np.int32),
+                    "cand_t": np.zeros(self.max_cand, np.int32),
+                    "cand_snr": np.zeros(self.max_cand, np.float32)}
+            o = ServerOutput(*[_ptr(outs[k], f32 if outs[k].dtype == np.float32 else i32)
+                               for k in ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr")])
+        self._live[tag] = (k0, k1, outs, o)
+        try:
+            call("vfrb_server_submit", self.h, sid, vp(a0), vp(a1), n, tag,
+                 C.byref(o) if o is not None else None)
+        except VfrbError:
+            del self._live[tag]
+            raise
+
+    def poll(self, wait_ms=-1, max_results=64):
+        res = (ServerResult * max_results)()
+        n = i32(0)
+        call("vfrb_server_poll", self.h, res, max_results, wait_ms, C.byref(n))
+        out = []
+        for r in res[: n.value]:
+            d = {k: getattr(r, k) for k, _ in ServerResult._fields_}
+            _, _, outs, _ = self._live.pop(r.tag)
+            if outs is not None:
+                k = min(r.ncand, self.max_cand)
+                d.update({"peak": outs["peak"], "peak_t": outs["peak_t"], "ave": outs["ave"],
+                          "sigma": outs["sigma"], "cand_dm": outs["cand_dm"][:k],
+                          "cand_t": outs["cand_t"][:k], "cand_snr": outs["cand_snr"][:k]})
+            out.append(d)
+        return out
+
+    def drain(self):
+        call("vfrb_server_drain", self.h)
+
+    def stats(self):
+        st = ServerStats()
+        call("vfrb_server_get_stats", self.h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def now(self):
+        return lib.vfrb_server_now(self.h)
+
+    def close(self):
+        if self.h:
+            lib.vfrb_server_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
