@@ -59,6 +59,10 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--payload-cache", default="",
                     help="file to reuse the synthesised payload between runs (written if absent)")
+    ap.add_argument("--sections", default="precision,batching,serving",
+                    help="extra measurements reported in the same line (empty = none)")
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="wall seconds of each replayed arrival trace")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -109,7 +113,7 @@ def main():
     d0 = vfrb.DeviceArray(nblk * n, np.uint8)
     d1 = vfrb.DeviceArray(nblk * n, np.uint8)
     chunk = max(1, (1 << 28) // n)  # 256 MB host chunks
-    host_first = {}
+    host_payload = {} if args.no_cpu_baseline else {0: np.empty(nblk * n, np.uint8), 1: np.empty(nblk * n, np.uint8)}
     cache = args.payload_cache
     reuse = bool(cache) and os.path.exists(cache) and os.path.getsize(cache) == 2 * nblk * n
     fcache = open(cache, "rb" if reuse else "wb") if cache else None
@@ -122,8 +126,8 @@ def main():
                 h = vfrb.synth_payload(spec, pol, b0, nb)
                 if fcache:
                     h.tofile(fcache)
-            if b0 == 0:
-                host_first[pol] = h[: min(h.size, (int(1.0 / tchan) + 1) * nave * n)].copy()
+            if host_payload:
+                host_payload[pol][b0 * n: (b0 + nb) * n] = h
             vfrb.call("vfrb_memcpy_h2d", d.offset(b0 * n), h.ctypes.data_as(C.c_void_p), h.nbytes, None)
     if fcache:
         fcache.close()
@@ -254,10 +258,23 @@ def main():
             e["TFLOPs"] = work[k]["flops"] / d / 1e12
         stages_roof[k] = e
 
-    # ---- CPU baseline: the oracle's FP32 stage mirror on a 1 s sample
+    # free the step's device buffers before the extra sections
+    del d0, d1, ring, out, cdm, ct, csnr
+    extra = {}
+    for sec in [x for x in args.sections.split(",") if x]:
+        fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving}[sec]
+        t = time.perf_counter()
+        try:
+            extra[sec] = fn(vfrb, args)
+        except Exception as e:  # a failed section is reported, the headline stands
+            log(f"section {sec} failed: {e!r}")
+            extra[sec] = {"error": repr(e)}
+        log(f"section {sec}: {time.perf_counter() - t:.1f} s")
+
+    # ---- CPU baseline: the oracle's FP32 stage mirror on one step
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args, n, nave, nch, tchan, dms, off, max_delay, seg_rows, win_rows, host_first)
+        cpu = cpu_baseline(args, n, nave, nch, tchan, dms, off, max_delay, seg_rows, win_rows, host_payload)
 
     line = {
         "metric": "real-time factor (data-s/s) + p99 latency, 4096ch×2048 DM, FP32 vs FP16",
@@ -292,48 +309,300 @@ def main():
         "cpu_baseline": cpu,
         "device": dev,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(args, n, nave, nch, tchan, dms, off, max_delay, seg_rows, win_rows, host_first):
-    """The oracle's FP32 stage mirror timed on this host on 1 s of the same
-    workload: filterbank of 976 new rows (from the first second of the
-    synthesised payload), bandpass over the rows a 1 s share of the window
-    carries, dedispersion of 976 output samples over a (976 + max_delay)-row
-    window, and the per-DM statistics."""
+def c1_shape(vfrb):
+    """1024 channels (2048-point FFT), 64 spectra averaged: 1.024 ms samples."""
+    n, nave = 2048, 64
+    p = vfrb.params_derive(n, nave, 1, 1.0, fullband=True)
+    return n, nave, p
+
+
+def bench_precision(vfrb, args):
+    """Precision variant of the 1 s, 1024 ch x 256 DM workload (one pulse at
+    DM 12.3, FWHM 2 ms, SNR 20, seed 0): the GPU chain in f32, f16 and u8
+    storage against the oracle's FP64 chain on the same payload.  Reports the
+    relative error of each DM trial's peak SNR and whether the candidate
+    (DM, sample) lists at 7 sigma are identical."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    n, nave, p = c1_shape(vfrb)
+    nch, tchan = p.nchanout, p.tchan
+    rows = int(1.0 / tchan)
+    dms = (np.arange(256) * 0.1).astype(np.float32)
+    amp = vfrb.amp_for_snr(20.0, nave, nch)
+    spec = vfrb.synth_spec(n, seed=0, pulses=[(12.3, 0.35, 0.002, amp)])
+    p0 = vfrb.synth_payload(spec, 0, 0, rows * nave)
+    p1 = vfrb.synth_payload(spec, 1, 0, rows * nave)
+    # FP64 oracle chain
+    fb64 = orc.filterbank_f64(p0, p1, rows, n, nave, p.nskip, nch)
+    off, max_delay, margin = vfrb.delay_table(dms, nch, p.fch1, p.chanbw, tchan)
+    nt = rows - max_delay
+    out64 = orc.dedisperse_f64(orc.bandpass_f64(fb64), off, nt)
+    ave64, sig64, peak64, pt64 = orc.dm_stats_f64(out64)
+    snr64 = (out64 - ave64[:, None]) / sig64[:, None]
+    thr = 7.0
+    ref_set = set(zip(*np.nonzero(snr64 > thr)))
+    near = float(np.min(np.abs(snr64 - thr)))
+    # GPU chain
+    d0, d1 = vfrb.DeviceArray.from_host(p0), vfrb.DeviceArray.from_host(p1)
+    fb = vfrb.DeviceArray((rows, nch), np.float32)
+    vfrb.FilterbankPlan(n, nave, p.nskip, nch).execute(d0, d1, rows, fb)
+    res = {"workload": "1.0 s, 1024 ch x 256 DM (0-25.5), pulse DM 12.3 SNR 20, seed 0",
+           "threshold": thr, "fp64_candidates": len(ref_set),
+           "fp64_min_distance_to_threshold": near, "delay_table_min_margin": margin,
+           "fp64_best": {"dm": float(dms[int(np.argmax(peak64))]), "snr": float(np.max(peak64))}}
+    for sname, acc in (("f32", vfrb.ACC_F64), ("f16", vfrb.ACC_F32C), ("u8", vfrb.ACC_F32C)):
+        sp = vfrb.SearchPlan(nch, tchan, p.fch1, p.chanbw, dms, rows, vfrb.STORAGE_NAMES[sname])
+        sp.load(fb, rows, bandpass=True)
+        out = vfrb.DeviceArray((len(dms), nt), np.float32)
+        sp.dedisperse(rows, out, nt)
+        st = vfrb.dm_stats(out, nt, len(dms), nt, acc=acc)
+        cnt, cdm, ct, _ = vfrb.candidates(out, nt, len(dms), nt, st.ave, st.sigma, thr, max_cand=1 << 16)
+        got = set(zip(cdm.tolist(), ct.tolist()))
+        rel = np.abs(st.peak.astype(np.float64) - peak64) / peak64
+        res[sname] = {"accumulate": "f64" if acc == vfrb.ACC_F64 else "f32 compensated",
+                      "peak_snr_rel_err_max": float(rel.max()),
+                      "peak_snr_rel_err_p50": float(np.median(rel)),
+                      "peak_snr_rel_err_at_best": float(rel[int(np.argmax(peak64))]),
+                      "peak_t_equal_fraction": float(np.mean(st.peak_t == pt64)),
+                      "candidates": cnt, "candidates_identical": got == ref_set,
+                      "missing": len(ref_set - got), "extra": len(got - ref_set)}
+        log(f"precision {sname}: max rel SNR err {rel.max():.2e} (median {np.median(rel):.2e}), "
+            f"candidates {cnt} identical={got == ref_set}")
+    return res
+
+
+def bench_batching(vfrb, args):
+    """16 antenna windows of uneven length (segments 2-20 s plus each stream's
+    max-delay overlap) with their own DM ranges (max DM 200-1000, 512-2048
+    trials), 1024 channels at 1.024 ms, one dispersed pulse each.  The
+    dedispersion of all 16 windows: one launch per window vs one persistent
+    batched launch (eager, and replayed from a captured hipGraph), the tile
+    picked by the batch autotuner.  Inputs are filterbank windows already
+    loaded (bandpassed and packed) in HBM."""
+    n, nave, p = c1_shape(vfrb)
+    nch, tchan = p.nchanout, p.tchan
+    rng = np.random.default_rng(4)
+    streams = []
+    for i in range(16):
+        seg_s = rng.uniform(2.0, 20.0)
+        dmax = rng.uniform(200.0, 1000.0)
+        ndm = int(rng.integers(512, 2049))
+        dms = np.linspace(0.0, dmax, ndm).astype(np.float32)
+        off, maxd, _ = vfrb.delay_table(dms, nch, p.fch1, p.chanbw, tchan)
+        seg_rows = int(seg_s / tchan)
+        rows = seg_rows + maxd
+        fb = rng.standard_normal((rows, nch), dtype=np.float32) * 22.0 + 256.0
+        d = int(ndm * 0.6)
+        t0 = seg_rows // 3
+        for c in range(nch):  # a 2-row pulse at DM index d, SNR ~20 after dedispersion
+            fb[t0 + off[d, c]: t0 + off[d, c] + 2, c] += 13.75
+        streams.append(dict(dms=dms, seg_rows=seg_rows, rows=rows, fb=vfrb.DeviceArray.from_host(fb),
+                            pulse=(d, t0)))
+        del fb
+    data_s = sum(x["seg_rows"] for x in streams) * tchan
+    s = vfrb.Stream()
+    ev0, ev1 = vfrb.Event(), vfrb.Event()
+
+    def timed(fn, reps=3):
+        fn()
+        ev0.record(s.h)
+        for _ in range(reps):
+            fn()
+        ev1.record(s.h)
+        ev1.sync()
+        return ev0.elapsed_ms(ev1) / reps
+
+    res = {"workload": "16 windows, segments U(2,20) s, max DM U(200,1000), 512-2048 DMs, 1024 ch",
+           "data_s": data_s, "windows": [{"segment_rows": x["seg_rows"], "ndm": len(x["dms"]),
+                                          "dm_max": float(x["dms"][-1])} for x in streams]}
+    for sname in ("f32", "f16"):
+        jobs = []
+        for x in streams:
+            sp = vfrb.SearchPlan(nch, tchan, p.fch1, p.chanbw, x["dms"], x["rows"], vfrb.STORAGE_NAMES[sname])
+            sp.load(x["fb"], x["rows"], bandpass=True, stream=s.h)
+            out = vfrb.DeviceArray((len(x["dms"]), x["seg_rows"]), np.float32)
+            jobs.append((sp, x["rows"], out, x["seg_rows"]))
+        b = vfrb.Batch(vfrb.STORAGE_NAMES[sname], max_jobs=16)
+        b.set(jobs, stream=s.h)
+        tune = b.autotune(iters=2, stream=s.h)
+        tile = vfrb.tile_names()[b.tile]
+        seq = timed(lambda: [sp.dedisperse(r, o, ld, stream=s.h) for sp, r, o, ld in jobs])
+        eager = timed(lambda: b.dedisperse(stream=s.h))
+        graph = timed(lambda: b.graph_launch(s.h))
+        # the pulse of every window is found at its DM in the batched output
+        found = 0
+        for (sp, r, o, ld), x in zip(jobs, streams):
+            d, t0 = x["pulse"]
+            found += int(np.argmax(o.download()[d]) in (t0, t0 + 1))
+        res[sname] = {"tile": tile, "grid": b.grid, "sequential_ms": seq, "batched_ms": eager,
+                      "graph_ms": graph, "speedup_graph_vs_sequential": seq / graph,
+                      "dedisp_rtf_sequential": data_s / (seq * 1e-3), "dedisp_rtf_graph": data_s / (graph * 1e-3),
+                      "pulses_found": found, "autotune_ms": tune}
+        log(f"batching {sname}: tile {tile} sequential {seq:.2f} ms, batched {eager:.2f} ms, graph "
+            f"{graph:.2f} ms ({seq / graph:.2f}x), pulses found {found}/16")
+        del b, jobs
+    return res
+
+
+def bench_serving(vfrb, args):
+    """Request loop at the 1024 ch x 256 DM shape: 16 streams submit segments
+    of U(0.5, 10) s with Poisson arrivals at aggregate offered loads of 0.5,
+    1 and 2 x real time for all 16 streams (8, 16, 32 data-s/s), replayed in
+    real time against the FP16 continuous-batching server and against the
+    plain FP32 path (one segment at a time) on the same trace; then every
+    segment of a trace submitted at once (closed loop) for the capacity.
+    Payload: segments are slices of a pinned 10.5 s host pool built from one
+    synthesised second (pulse at DM 12.3), so arrival-to-candidate latency
+    includes the host-to-device copy."""
+    n, nave, p = c1_shape(vfrb)
+    nch, tchan = p.nchanout, p.tchan
+    dms = (np.arange(256) * 0.1).astype(np.float32)
+    rate = 128e6  # bytes per pol per data-second
+    sec = int(rate)
+    spec = vfrb.synth_spec(n, seed=0, pulses=[(12.3, 0.35, 0.002, vfrb.amp_for_snr(20.0, nave, nch))])
+    pools = []
+    pool_s = 10.5
+    for pol in (0, 1):
+        one = vfrb.synth_payload(spec, pol, 0, sec // n)
+        pb = vfrb.PinnedBytes(int(pool_s * sec))
+        reps = -(-pb.array.size // one.size)
+        for k in range(reps):
+            a = k * one.size
+            m = min(one.size, pb.array.size - a)
+            pb.array[a: a + m] = one[:m]
+        pools.append(pb)
+    dpools = [vfrb.DeviceArray.from_host(pb.array) for pb in pools]
+    rng = np.random.default_rng(5)
+    dur = args.serve_seconds
+
+    def trace(load):
+        lam = load * 16.0 / 5.25  # arrivals per second (mean length 5.25 s)
+        t, ev = 0.0, []
+        while True:
+            t += rng.exponential(1.0 / lam)
+            if t > dur:
+                return ev
+            length = rng.uniform(0.5, 10.0)
+            nbytes = int(length * rate)
+            off = int(rng.integers(0, pools[0].array.size - nbytes))
+            ev.append((t, int(rng.integers(0, 16)), off, nbytes))
+
+    def make(batched, storage):
+        srv = vfrb.Server(n, nave, nch, p.nskip, p.fch1, p.chanbw, tchan, storage=storage,
+                          acc=vfrb.ACC_F32C if storage != vfrb.STORE_F32 else vfrb.ACC_F64,
+                          threshold=7.0, max_cand=4096, max_streams=16,
+                          max_segment_bytes=int(10.0 * rate) + 1, arena_bytes=48 << 30,
+                          max_batch=16, batched=batched)
+        ids = [srv.open_stream(dms) for _ in range(16)]
+        return srv, ids
+
+    def replay(srv, ids, ev, realtime=True, device=True):
+        t0 = time.perf_counter()
+        got = []
+        for k, (t, sidx, off, nb) in enumerate(ev):
+            if realtime:
+                while time.perf_counter() - t0 < t:
+                    got += srv.poll(wait_ms=1)
+            if device:
+                srv.submit_device(ids[sidx], dpools[0], dpools[1], off, nb, k, outputs=False)
+            else:
+                srv.submit(ids[sidx], pools[0].array[off: off + nb], pools[1].array[off: off + nb], k,
+                           outputs=False)
+        srv.drain()
+        wall = time.perf_counter() - t0
+        got += srv.poll(wait_ms=0, max_results=4096)
+        lat = np.array([r["done_s"] - r["arrival_s"] for r in got])
+        seglen = np.array([ev[r["tag"]][3] / rate for r in got])
+        bs = np.array([r["batch_size"] for r in got])
+        st = srv.stats()
+        return {"segments": len(got), "data_s": float(seglen.sum()), "wall_s": wall,
+                "throughput_data_s_per_s": float(seglen.sum() / wall),
+                "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                "max_latency_ms": float(lat.max() * 1e3),
+                "p99_latency_over_segment": float(np.percentile(lat / seglen, 99)),
+                "mean_batch": float(bs.mean()), "iterations": st["iterations"],
+                "device_busy_s": st["device_busy_s"], "errors": int(sum(r["status"] != 0 for r in got))}
+
+    res = {"workload": "16 streams, 1024 ch x 256 DM, segments U(0.5,10) s, Poisson arrivals",
+           "trace_wall_s": dur,
+           "input": {"device": "payload resident in HBM (NIC-to-GPU path; primary)",
+                     "host": "payload in pinned host memory, copied per segment (PCIe-inclusive)"}}
+    modes = (("fp16_batched", True, vfrb.STORE_F16), ("fp32_plain", False, vfrb.STORE_F32))
+    for inp in ("device", "host"):
+        for load in (0.5, 1.0, 2.0):
+            ev = trace(load)
+            for name, batched, storage in modes:
+                srv, ids = make(batched, storage)
+                r = replay(srv, ids, ev, device=inp == "device")
+                srv.close()
+                res.setdefault(inp, {}).setdefault(f"load_{load:g}x", {})[name] = r
+                log(f"serving {inp} load {load}x {name}: {r['segments']} segs, p50 "
+                    f"{r['p50_latency_ms']:.1f} ms, p99 {r['p99_latency_ms']:.1f} ms, thr "
+                    f"{r['throughput_data_s_per_s']:.1f} data-s/s, mean batch {r['mean_batch']:.2f}")
+        # capacity: a 2x trace submitted at once
+        ev = trace(2.0)
+        for name, batched, storage in modes:
+            srv, ids = make(batched, storage)
+            r = replay(srv, ids, ev, realtime=False, device=inp == "device")
+            srv.close()
+            res[inp].setdefault("closed_loop", {})[name] = r
+            log(f"serving {inp} closed loop {name}: {r['throughput_data_s_per_s']:.1f} data-s/s, "
+                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+        c = res[inp]["closed_loop"]
+        res[inp]["capacity_ratio_fp16_batched_vs_fp32_plain"] = (
+            c["fp16_batched"]["throughput_data_s_per_s"] / c["fp32_plain"]["throughput_data_s_per_s"])
+    return res
+
+
+def cpu_baseline(args, n, nave, nch, tchan, dms, off, max_delay, seg_rows, win_rows, host_payload):
+    """The oracle's FP32 stage mirror timed on this host on one step of the
+    same workload: filterbank of the segment's payload, bandpass over the
+    (max_delay + segment)-row window, transposition to channel-major,
+    dedispersion of the segment's output samples, per-DM statistics.  The
+    carried overlap rows are stood in for by the segment's last rows (same
+    size and cost)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
     if args.cpu_threads > 0:
         orc.set_threads(args.cpu_threads)
     threads = orc.num_threads()
-    rows = int(1.0 / tchan)
-    p0, p1 = host_first[0], host_first[1]
+    rows = seg_rows
+    p0, p1 = host_payload[0], host_payload[1]
     t0 = time.perf_counter()
     fb = orc.filterbank_f32(p0, p1, rows, n, nave, 1, nch)
     t_fb = time.perf_counter() - t0
-    bp_rows = int(round(rows * win_rows / seg_rows))
-    rng = np.random.default_rng(0)
-    fbw = rng.normal(256, 45, (bp_rows, nch)).astype(np.float32)
+    window = np.concatenate([fb[rows - max_delay:], fb])
+    del fb
     t0 = time.perf_counter()
-    orc.bandpass_f32(fbw)
+    bp = orc.bandpass_f32(window)
     t_bp = time.perf_counter() - t0
-    wrows = rows + max_delay
-    fbcm = rng.normal(0, 45, (nch, wrows)).astype(np.float32)
+    t0 = time.perf_counter()
+    fbcm = np.ascontiguousarray(bp.T)
+    t_tr = time.perf_counter() - t0
+    del bp, window
     t0 = time.perf_counter()
     dd = orc.dedisperse_cm_f32(fbcm, off, rows)
     t_dd = time.perf_counter() - t0
     t0 = time.perf_counter()
     orc.dm_stats(dd)
     t_st = time.perf_counter() - t0
-    total = t_fb + t_bp + t_dd + t_st
-    log(f"cpu baseline ({threads} threads): fb={t_fb:.2f}s bp={t_bp:.2f}s dedisp={t_dd:.2f}s "
-        f"stats={t_st:.2f}s -> RTF {rows * tchan / total:.3f}")
+    total = t_fb + t_bp + t_tr + t_dd + t_st
+    log(f"cpu baseline ({threads} threads): fb={t_fb:.2f}s bp={t_bp:.2f}s transpose={t_tr:.2f}s "
+        f"dedisp={t_dd:.2f}s stats={t_st:.2f}s -> RTF {rows * tchan / total:.3f}")
     return {"value": rows * tchan / total, "unit": "data-s/s", "cores": threads, "kind": "port",
-            "sample": f"1 s of the same workload ({rows} rows): FP32 stage-mirror filterbank of the first "
-                      f"second of the payload, bandpass over {bp_rows} rows, dedispersion of {rows} samples x "
-                      f"{len(dms)} DMs x {nch} ch over a {wrows}-row window, per-DM stats",
-            "stage_s": {"filterbank": t_fb, "bandpass": t_bp, "dedisperse": t_dd, "dm_stats": t_st}}
+            "sample": f"one step of the same workload ({rows} new rows = {rows * tchan:.2f} data-s): FP32 "
+                      f"stage-mirror filterbank of the segment payload, bandpass over the {win_rows}-row "
+                      f"window, transpose, dedispersion of {rows} samples x {len(dms)} DMs x {nch} ch, "
+                      f"per-DM stats; {total:.1f} s of CPU work",
+            "stage_s": {"filterbank": t_fb, "bandpass": t_bp, "transpose": t_tr, "dedisperse": t_dd,
+                        "dm_stats": t_st}}
 
 
 if __name__ == "__main__":

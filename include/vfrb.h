@@ -424,6 +424,11 @@ int32_t     vfrb_server_max_delay(const vfrb_server* server, int32_t stream_id);
 vfrb_status vfrb_server_submit(vfrb_server* server, int32_t stream_id, const uint8_t* pol0,
                                const uint8_t* pol1, int64_t nbytes, int64_t tag,
                                const vfrb_server_output* out);
+/* Same, with the payload already in device memory (a NIC-to-GPU path):
+ * no host-to-device copy; pol0/pol1 stay valid until the result is polled. */
+vfrb_status vfrb_server_submit_device(vfrb_server* server, int32_t stream_id, const uint8_t* pol0,
+                                      const uint8_t* pol1, int64_t nbytes, int64_t tag,
+                                      const vfrb_server_output* out);
 /*
  * Moves up to max_results finished results (completion order) into results;
  * waits up to wait_ms (< 0: until one is ready or nothing is outstanding) when

@@ -34,14 +34,22 @@ def _server(vfrb, p, storage, batched, **kw):
                        arena_bytes=kw.pop("arena_bytes", 1 << 28), max_batch=8, batched=batched, **kw)
 
 
-def _run(srv, streams):
+def _run(srv, streams, device=False):
     """streams: list of (sid, p0, p1, cuts, tag_base); submits segment k of
-    every stream before segment k+1 of any, then collects all results."""
+    every stream before segment k+1 of any, then collects all results.
+    device=True uploads each stream's payload once and submits device slices."""
     nseg = len(SEG_ROWS)
+    if device:
+        import vfrb
+        dev = {sid: (vfrb.DeviceArray.from_host(p0), vfrb.DeviceArray.from_host(p1))
+               for sid, p0, p1, _, _ in streams}
     for k in range(nseg):
         for sid, p0, p1, cuts, base in streams:
             a, b = cuts[k], cuts[k + 1]
-            srv.submit(sid, p0[a:b], p1[a:b], base + k)
+            if device:
+                srv.submit_device(sid, dev[sid][0], dev[sid][1], int(a), int(b - a), base + k)
+            else:
+                srv.submit(sid, p0[a:b], p1[a:b], base + k)
     srv.drain()
     res = {}
     while True:
@@ -110,6 +118,11 @@ def test_server_result_is_the_same_alone_and_in_the_mix(gpu):
     sid = s2.open_stream(DMS)
     plain = _run(s2, [(sid, *a, 0)])
     s2.close()
+    print("[case] server f16: stream alone (batched, device-resident payload)", flush=True)
+    s4 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    sid = s4.open_stream(DMS)
+    dev = _run(s4, [(sid, *a, 0)], device=True)
+    s4.close()
     print("[case] server f16: stream in a mix of four", flush=True)
     s3 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
     ids = [s3.open_stream(DMS) for _ in range(4)]
@@ -124,7 +137,7 @@ def test_server_result_is_the_same_alone_and_in_the_mix(gpu):
     keys = ("status", "rows", "sample0", "nt", "ncand", "best_dm", "best_t", "best_snr")
     arrays = ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr")
     for k in range(len(SEG_ROWS)):
-        for other in (plain[k], mix[k]):
+        for other in (plain[k], mix[k], dev[k]):
             for key in keys:
                 assert alone[k][key] == other[key], (k, key)
             if alone[k]["nt"]:

@@ -94,6 +94,7 @@ struct Request {
     int stream = 0;
     const uint8_t* pol[2] = {nullptr, nullptr};
     int64_t nbytes = 0;
+    bool device = false;  // payload already in device memory
     vfrb_server_output out{};
     double arrival = 0.0;
 };
@@ -200,7 +201,9 @@ void issue(vfrb_server* S, Lane& L) {
         vfrb_status e = VFRB_OK;
         for (int p = 0; p < 2 && e == VFRB_OK; p++) {
             if (C > 0) e = vfrb_memcpy_d2d(reg[p], st->carry + p * rb, (size_t)C, s);
-            if (e == VFRB_OK) e = vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+            if (e == VFRB_OK)
+                e = it.req.device ? vfrb_memcpy_d2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s)
+                                  : vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
         }
         if (e == VFRB_OK && R > 0)
             e = vfrb_fb_execute(S->fbp, reg[0], reg[1], R, st->ring, st->ring_rows, st->total_rows, s);
@@ -565,9 +568,8 @@ int32_t vfrb_server_max_delay(const vfrb_server* S, int32_t id) {
     return S->streams[id]->max_delay;
 }
 
-vfrb_status vfrb_server_submit(vfrb_server* S, int32_t stream_id, const uint8_t* pol0,
-                               const uint8_t* pol1, int64_t nbytes, int64_t tag,
-                               const vfrb_server_output* out) {
+static vfrb_status submit(vfrb_server* S, int32_t stream_id, const uint8_t* pol0, const uint8_t* pol1,
+                          int64_t nbytes, int64_t tag, const vfrb_server_output* out, bool device) {
     VFRB_REQUIRE(S && pol0 && pol1 && nbytes > 0, VFRB_ERR_INVALID, "bad segment");
     VFRB_REQUIRE(nbytes <= S->d.max_segment_bytes, VFRB_ERR_CAPACITY,
                  "segment of %lld bytes exceeds max_segment_bytes %lld", (long long)nbytes,
@@ -580,6 +582,7 @@ vfrb_status vfrb_server_submit(vfrb_server* S, int32_t stream_id, const uint8_t*
     r.pol[0] = pol0;
     r.pol[1] = pol1;
     r.nbytes = nbytes;
+    r.device = device;
     if (out) r.out = *out;
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -591,6 +594,18 @@ vfrb_status vfrb_server_submit(vfrb_server* S, int32_t stream_id, const uint8_t*
     }
     S->cv_sched.notify_one();
     return VFRB_OK;
+}
+
+vfrb_status vfrb_server_submit(vfrb_server* S, int32_t stream_id, const uint8_t* pol0,
+                               const uint8_t* pol1, int64_t nbytes, int64_t tag,
+                               const vfrb_server_output* out) {
+    return submit(S, stream_id, pol0, pol1, nbytes, tag, out, false);
+}
+
+vfrb_status vfrb_server_submit_device(vfrb_server* S, int32_t stream_id, const uint8_t* pol0,
+                                      const uint8_t* pol1, int64_t nbytes, int64_t tag,
+                                      const vfrb_server_output* out) {
+    return submit(S, stream_id, pol0, pol1, nbytes, tag, out, true);
 }
 
 vfrb_status vfrb_server_poll(vfrb_server* S, vfrb_server_result* results, int32_t max_results,

@@ -155,6 +155,7 @@ _SIGS = {
     "vfrb_server_open_stream": (i32, [vp, P(f32), i32, P(i32)]),
     "vfrb_server_max_delay": (i32, [vp, i32]),
     "vfrb_server_submit": (i32, [vp, i32, vp, vp, i64, i64, P(ServerOutput)]),
+    "vfrb_server_submit_device": (i32, [vp, i32, vp, vp, i64, i64, P(ServerOutput)]),
     "vfrb_server_poll": (i32, [vp, P(ServerResult), i32, i32, P(i32)]),
     "vfrb_server_drain": (i32, [vp]),
     "vfrb_server_get_stats": (i32, [vp, P(ServerStats)]),
@@ -609,11 +610,22 @@ class Server:
 
     def submit(self, sid, pol0, pol1, tag, outputs=True):
         """pol0/pol1: contiguous uint8 arrays of the same length (views of
-        pinned_bytes() memory copy fastest); referenced until polled."""
+        PinnedBytes memory copy fastest); referenced until polled."""
         assert pol0.dtype == np.uint8 and pol1.dtype == np.uint8 and len(pol0) == len(pol1)
         assert pol0.flags.c_contiguous and pol1.flags.c_contiguous
-        k0, k1, n = pol0, pol1, len(pol0)
-        a0, a1 = pol0.ctypes.data, pol1.ctypes.data
+        self._submit("vfrb_server_submit", sid, pol0.ctypes.data, pol1.ctypes.data, len(pol0),
+                     (pol0, pol1), tag, outputs)
+
+    def submit_device(self, sid, dev0: "DeviceArray", dev1: "DeviceArray", offset, nbytes, tag,
+                      outputs=True):
+        """Payload already in device memory: bytes [offset, offset+nbytes) of
+        dev0/dev1 (kept alive until polled)."""
+        assert offset >= 0 and offset + nbytes <= min(dev0.nbytes, dev1.nbytes)
+        self._submit("vfrb_server_submit_device", sid, dev0.ptr.value + offset,
+                     dev1.ptr.value + offset, nbytes, (dev0, dev1), tag, outputs)
+
+    def _submit(self, fn, sid, a0, a1, n, keep, tag, outputs):
+        k0, k1 = keep
         ndm = self._ndm.get(sid, 1)  # unknown ids are rejected by the library
         outs = None
         o = None
@@ -627,8 +639,7 @@ class Server:
                                for k in ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr")])
         self._live[tag] = (k0, k1, outs, o)
         try:
-            call("vfrb_server_submit", self.h, sid, vp(a0), vp(a1), n, tag,
-                 C.byref(o) if o is not None else None)
+            call(fn, self.h, sid, vp(a0), vp(a1), n, tag, C.byref(o) if o is not None else None)
         except VfrbError:
             del self._live[tag]
             raise
