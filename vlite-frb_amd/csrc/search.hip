@@ -210,7 +210,7 @@ struct Chunk {
 // and two fit in one CU's 160 KB of LDS.
 constexpr int kChunkMax = 32;
 constexpr size_t kDataBudget = 60 * 1024;
-constexpr int kStageU = 8;  // staging loads in flight per lane
+constexpr int kStageG = 2;  // 4-dword staging groups in flight per lane
 
 // LDS (<= 76 KB per workgroup) limits a CU to two workgroups, i.e. W/2 waves
 // per SIMD: let the register allocator use the VGPR budget that occupancy
@@ -259,30 +259,44 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     for (int k = 0; k < cr.y; k++) {
         const Chunk ch = chunks[cr.x + k];
         const int rowb = 4 * ch.stride;    // bytes per staged channel (all copies)
-        const int per_row = ch.stride + EPD - 1;
-        // stage copy p: element i = window[c][t0 + dmin_c + i + p], rows >= nrows -> 0;
-        // kStageU loads per lane are in flight before the LDS writes
+        const int nsd = ch.stride / EPD;   // dwords per copy, a multiple of 4
+        // Stage the channel's row from its dword-aligned start a0 = r0 - phase
+        // (the phase, dmin mod EPD, is folded into the offset table): lane l
+        // loads source dwords 4g..4g+4 of group g and writes dwords 4g..4g+3
+        // of every copy, copy p being the source shifted by p elements
+        // (v_alignbyte across neighbouring dwords).  Dwords past the row end
+        // are clamped to it: they only feed samples t >= nt, never stored.
         for (int ci = wave; ci < ch.count; ci += W) {
             const int c = ch.c0 + ci;
-            const int64_t src0 = (int64_t)c * ld;
             const int r0 = t0 + dmin[(int64_t)b * nch + c];
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(win + (int64_t)c * ld);
+            const int sd0 = r0 / EPD;
+            const int last = (int)(ld / EPD) - 1;
             unsigned char* row = data + ci * rowb;
-            for (int s0 = 0; s0 < per_row; s0 += 64 * kStageU) {
-                TS v[kStageU];
+            for (int g0 = 0; g0 < nsd / 4; g0 += 64 * kStageG) {
+                uint32_t sv[kStageG][5];
 #pragma unroll
-                for (int u = 0; u < kStageU; u++) {
-                    const int s = s0 + lane + 64 * u;
-                    const int r = r0 + s;
-                    v[u] = (s < per_row && r < nrows) ? win[src0 + r] : TS(0);
+                for (int u = 0; u < kStageG; u++) {
+                    const int j = sd0 + 4 * (g0 + lane + 64 * u);
+#pragma unroll
+                    for (int e = 0; e < (EPD > 1 ? 5 : 4); e++) sv[u][e] = src[min(j + e, last)];
                 }
 #pragma unroll
-                for (int u = 0; u < kStageU; u++) {
-                    const int s = s0 + lane + 64 * u;
+                for (int u = 0; u < kStageG; u++) {
+                    const int g = g0 + lane + 64 * u;
+                    if (4 * g >= nsd) continue;
 #pragma unroll
                     for (int p = 0; p < EPD; p++) {
-                        const int i = s - p;
-                        if (i >= 0 && i < ch.stride)
-                            reinterpret_cast<TS*>(row + p * ch.stride * ESZ)[i] = v[u];
+                        uint4 w;
+                        if (p == 0) {
+                            w = make_uint4(sv[u][0], sv[u][1], sv[u][2], sv[u][3]);
+                        } else {
+                            w.x = __builtin_amdgcn_alignbyte(sv[u][1], sv[u][0], p * ESZ);
+                            w.y = __builtin_amdgcn_alignbyte(sv[u][2], sv[u][1], p * ESZ);
+                            w.z = __builtin_amdgcn_alignbyte(sv[u][3], sv[u][2], p * ESZ);
+                            w.w = __builtin_amdgcn_alignbyte(sv[u][4], sv[u][3], p * ESZ);
+                        }
+                        *reinterpret_cast<uint4*>(row + p * ch.stride * ESZ + 16 * g) = w;
                     }
                 }
             }
@@ -358,7 +372,7 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
               const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
               const float* __restrict__ chan_scale, const int* __restrict__ f16_exp,
               const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     dedisp_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
                                chan_scale, f16_exp, offsum, out, ld_out, blockIdx.y, blockIdx.x,
                                lds_raw);
@@ -395,7 +409,7 @@ template <typename TS, int SL, int DW, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
 dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restrict__ tiles,
                     const int* __restrict__ ntiles, unsigned* __restrict__ ctr) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     __shared__ int s_tile;
     const int total = ntiles[0];
     for (;;) {
@@ -582,8 +596,11 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     std::vector<int2> blk(nblk);
     std::vector<int64_t> cost(nblk, 0);
     // A staged channel takes 4*stride bytes in every storage type: one f32
-    // copy, two shifted f16 copies or four shifted u8 copies.
-    auto stride_of = [&](int s) { return (T + s + 3) / 4 * 4; };
+    // copy, two shifted f16 copies or four shifted u8 copies.  A copy starts
+    // at the dword-aligned element below the first one needed (phase up to
+    // epd-1) and a read dword reaches epd-1 elements past its sample, so a
+    // copy holds T + span + 2*epd elements, rounded to 16 (16-byte rows).
+    auto stride_of = [&](int s) { return (T + s + 2 * epd + 15) / 16 * 16; };
     for (int b = 0; b < nblk; b++) {
         const int d0 = b * DMB, d1 = std::min(ndm, d0 + DMB);
         std::vector<int> span(nch);
@@ -614,10 +631,12 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
             chunks.push_back(Chunk{c, count, stride, (int)roffs.size()});
             for (int ci = 0; ci < count; ci++) {
                 const int cc = c + ci;
+                const int phase = dmin[(size_t)b * nch + cc] & (epd - 1);
                 for (int d = 0; d < DMB; d++) {
                     const int r = d0 + d < d1 ? p->off[(size_t)(d0 + d) * nch + cc] - dmin[(size_t)b * nch + cc] : 0;
-                    const int q = r & (epd - 1);
-                    roffs.push_back((uint32_t)(ci * 4 * stride + q * stride * esz + (r - q) * esz));
+                    const int e = r + phase;  // element index from the aligned start
+                    const int q = e & (epd - 1);
+                    roffs.push_back((uint32_t)(ci * 4 * stride + q * stride * esz + (e - q) * esz));
                 }
             }
             max_data = std::max(max_data, (size_t)count * 4 * stride);

@@ -116,6 +116,22 @@ vfrb_status vfrb_synth_payload(const vfrb_synth_spec* spec, int32_t pol, int64_t
         tone_phase[i] = 2.0 * M_PI * (double)(r >> 11) * (1.0 / 9007199254740992.0);
     }
     const float mean = (float)spec->mean, sig = (float)spec->sigma;
+    // Tones on integer bins repeat every block: their sum is tabulated once.
+    bool periodic = true;
+    for (int t = 0; t < spec->ntone; t++)
+        periodic = periodic && spec->tone_bin[t] == std::floor(spec->tone_bin[t]);
+    std::vector<float> tone_table;
+    if (spec->ntone > 0 && periodic) {
+        tone_table.assign(n, 0.0f);
+        std::vector<double> acc(n, 0.0);
+        for (int t = 0; t < spec->ntone; t++)
+            for (int64_t i = 0; i < n; i++) {
+                const double ph = 2.0 * M_PI * std::fmod(spec->tone_bin[t] * (double)i / (double)n, 1.0) +
+                                  tone_phase[t];
+                acc[i] += spec->tone_amp[t] * std::cos(ph);
+            }
+        for (int64_t i = 0; i < n; i++) tone_table[i] = (float)acc[i];
+    }
     std::vector<std::complex<double>> tw;
     if (spec->npulse > 0) {
         tw.resize(n / 2);
@@ -133,11 +149,22 @@ vfrb_status vfrb_synth_payload(const vfrb_synth_spec* spec, int32_t pol, int64_t
             x[i] = mean + sig * z0;
             x[i + 1] = mean + sig * z1;
         }
-        for (int t = 0; t < spec->ntone; t++) {
+        if (!tone_table.empty())
+            for (int64_t i = 0; i < n; i++) x[i] += tone_table[i];
+        // Other tones: a unit phasor rotated by exp(2*pi*i*f) per sample,
+        // re-anchored from the exact phase every 256 samples (drift ~1e-14).
+        for (int t = 0; t < spec->ntone && tone_table.empty(); t++) {
             const double f = spec->tone_bin[t] / (double)n;  // cycles per sample
-            for (int64_t i = 0; i < n; i++) {
-                const double ph = 2.0 * M_PI * std::fmod(f * (double)(s0 + i), 1.0) + tone_phase[t];
-                x[i] += (float)(spec->tone_amp[t] * std::cos(ph));
+            const double amp = spec->tone_amp[t];
+            const std::complex<double> step(std::cos(2.0 * M_PI * f), std::sin(2.0 * M_PI * f));
+            for (int64_t i0 = 0; i0 < n; i0 += 256) {
+                const double ph = 2.0 * M_PI * std::fmod(f * (double)(s0 + i0), 1.0) + tone_phase[t];
+                std::complex<double> z(std::cos(ph), std::sin(ph));
+                const int64_t i1 = std::min<int64_t>(n, i0 + 256);
+                for (int64_t i = i0; i < i1; i++) {
+                    x[i] += (float)(amp * z.real());
+                    z *= step;
+                }
             }
         }
         if (spec->npulse > 0) {
