@@ -33,6 +33,15 @@ __device__ __forceinline__ cf cscale(cf a, float s) { return {a.x * s, a.y * s};
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 
+// Additive LDS padding i + M*(i >> S): the constant strides of a pass stay
+// constant offsets, so the padded address folds into ds_read/ds_write
+// immediates.
+template <int S, int M>
+struct Pad {
+    __device__ static __forceinline__ int idx(int i) { return i + M * (i >> S); }
+    static constexpr int words(int n) { return n + M * ((n - 1) >> S) + 1; }
+};
+
 // Forward DFTs (kernel exp(-2*pi*i*n*k/R)) on registers.
 __device__ __forceinline__ void dft2(cf* v) {
     cf a = v[0], b = v[1];
@@ -203,17 +212,27 @@ struct P2 {
     static constexpr int BLOCK = TPF > 256 ? TPF : 256;
     static constexpr int FPB = BLOCK / TPF;
     static constexpr int TOT = FPB * N;
-    static constexpr int LDS_WORDS = TOT + TOT / 32;      // per component
     static constexpr int LOG2 = __builtin_ctz(N);
     static constexpr int MID_BITS = LOG2 - 7;             // between radix-8 and radix-16
     // middle radices (0 = unused): 8192 -> 8,8; 16384 -> 16,8; else one pass
     static constexpr int MID0 = MID_BITS == 6 ? 8 : MID_BITS == 7 ? 16 : (1 << MID_BITS);
     static constexpr int MID1 = MID_BITS == 6 ? 8 : MID_BITS == 7 ? 8 : 0;
+    // One padding per LDS generation (written by one pass, read by the
+    // next), each chosen by enumerating the wave access patterns of its
+    // writer and reader against 32 four-byte banks: A = pass 1 -> first
+    // middle pass, B = between the two middle passes, C = -> last pass.
+    using PadA = Pad<5, 1>;
+    using PadB = Pad<5, 2>;
+    using PadC = Pad<(N >= 8192 ? 8 : 5), (N >= 8192 ? 1 : N == 2048 ? 2 : N == 1024 ? 3 : 1)>;
+    static_assert(MID0 > 1, "the power-of-two kernel needs at least one middle pass");
+    static constexpr int cmax(int a, int b) { return a > b ? a : b; }
+    static constexpr int LDS_WORDS =                      // per component
+        cmax(PadA::words(TOT), cmax(MID1 > 1 ? PadB::words(TOT) : 0, PadC::words(TOT)));
 };
 
 // One radix-R Stockham pass through LDS; t = thread index within its FFT,
-// fb = element offset of that FFT in LDS.
-template <int N, int R>
+// fb = element offset of that FFT in LDS; reads in padding PR, writes in PW.
+template <int N, int R, class PR, class PW>
 __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restrict__ im, int Ns, int t,
                                         int fb, const float2* __restrict__ tw) {
     constexpr int TPF = P2<N>::TPF;
@@ -225,7 +244,7 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
         const int j = t + TPF * q;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int idx = lds_pad(fb + j + r * STRIDE);
+            const int idx = PR::idx(fb + j + r * STRIDE);
             v[q][r] = cf{re[idx], im[idx]};
         }
     }
@@ -240,7 +259,7 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int idx = lds_pad(fb + base + r * Ns);
+            const int idx = PW::idx(fb + base + r * Ns);
             re[idx] = v[q][r].x;
             im[idx] = v[q][r].y;
         }
@@ -250,7 +269,7 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
 
 __device__ __forceinline__ float norm2(cf a) { return a.x * a.x + a.y * a.y; }
 
-// LDS (66 KB for N <= 8192) allows two workgroups per CU, i.e. 2 waves per
+// LDS (70 KB for N <= 8192) allows two workgroups per CU, i.e. 2 waves per
 // SIMD: hold the register allocator to that budget.
 template <int N>
 __global__ void __launch_bounds__(P2<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -322,7 +341,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         for (int q = 0; q < 4; q++) {
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const int idx = lds_pad(fbase + 32 * t + 8 * q + r);
+                const int idx = C::PadA::idx(fbase + 32 * t + 8 * q + r);
                 re[idx] = v[q][r].x;
                 im[idx] = v[q][r].y;
             }
@@ -330,13 +349,14 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         __syncthreads();
         // ---- middle passes
         int Ns = 8;
-        if constexpr (C::MID0 > 1) {
-            p2_pass<N, C::MID0>(re, im, Ns, t, fbase, twf);
-            Ns *= C::MID0;
-        }
         if constexpr (C::MID1 > 1) {
-            p2_pass<N, C::MID1>(re, im, Ns, t, fbase, twf);
+            p2_pass<N, C::MID0, typename C::PadA, typename C::PadB>(re, im, Ns, t, fbase, twf);
+            Ns *= C::MID0;
+            p2_pass<N, C::MID1, typename C::PadB, typename C::PadC>(re, im, Ns, t, fbase, twf);
             Ns *= C::MID1;
+        } else if constexpr (C::MID0 > 1) {
+            p2_pass<N, C::MID0, typename C::PadA, typename C::PadC>(re, im, Ns, t, fbase, twf);
+            Ns *= C::MID0;
         }
         // ---- last pass: radix 16 on butterflies ja, jb (Ns == N/16); only
         // |Z|^2 is needed: na[r] = |Z[ja + r*NS]|^2, nb[r] = |Z[jb + r*NS]|^2
@@ -345,7 +365,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             cf a[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int ia = lds_pad(fbase + ja + r * NS);
+                const int ia = C::PadC::idx(fbase + ja + r * NS);
                 a[r] = cf{re[ia], im[ia]};
             }
             apply_twiddles<16>(a, twf, ja);
@@ -357,7 +377,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             cf b[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int ib = lds_pad(fbase + jb + r * NS);
+                const int ib = C::PadC::idx(fbase + jb + r * NS);
                 b[r] = cf{re[ib], im[ib]};
             }
             apply_twiddles<16>(b, twf, jb);
