@@ -38,7 +38,11 @@ __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 5); }
 // immediates.
 template <int S, int M>
 struct Pad {
+    static constexpr int SPAN = 1 << S;
     __device__ static __forceinline__ int idx(int i) { return i + M * (i >> S); }
+    // idx(x + c) == idx(x) + off(c) whenever (x mod SPAN) + (c mod SPAN) <
+    // SPAN: a thread's padded base plus a constant, i.e. an immediate offset.
+    static constexpr int off(int c) { return c + M * (c >> S); }
     static constexpr int words(int n) { return n + M * ((n - 1) >> S) + 1; }
 };
 
@@ -230,36 +234,48 @@ struct P2 {
         cmax(PadA::words(TOT), cmax(MID1 > 1 ? PadB::words(TOT) : 0, PadC::words(TOT)));
 };
 
-// One radix-R Stockham pass through LDS; t = thread index within its FFT,
-// fb = element offset of that FFT in LDS; reads in padding PR, writes in PW.
-template <int N, int R, class PR, class PW>
-__device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restrict__ im, int Ns, int t,
+// One radix-R Stockham pass through LDS (NS = product of the radices
+// before it); t = thread index within its FFT, fb = element offset of that
+// FFT in LDS (a multiple of N); reads in padding PR, writes in PW.  Where
+// the constant part of an index cannot carry into the padding's shift
+// (checked at compile time), the address is the thread's padded base plus
+// an immediate.
+template <int N, int R, int NS, class PR, class PW>
+__device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restrict__ im, int t,
                                         int fb, const float2* __restrict__ tw) {
     constexpr int TPF = P2<N>::TPF;
     constexpr int Q = 32 / R;
     constexpr int STRIDE = N / R;
+    // reads fb + t + (TPF*q + r*STRIDE): the constant is a multiple of SPAN
+    constexpr bool FOLD_RD = TPF % PR::SPAN == 0 && STRIDE % PR::SPAN == 0;
+    // writes fb + (t-k)*R + k + (TPF*q*R + r*NS), k = t mod NS < NS: the
+    // base is k mod SPAN, the constant's residue at most SPAN - NS
+    constexpr bool FOLD_WR = TPF % NS == 0 && (NS * R) % PW::SPAN == 0 && (TPF * R) % PW::SPAN == 0;
     cf v[Q][R];
+    const int xr = PR::idx(fb + t);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int j = t + TPF * q;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int idx = PR::idx(fb + j + r * STRIDE);
+            const int idx = FOLD_RD ? xr + PR::off(TPF * q + r * STRIDE) : PR::idx(fb + j + r * STRIDE);
             v[q][r] = cf{re[idx], im[idx]};
         }
     }
     __syncthreads();
-    const int tw_shift = P2<N>::LOG2 - __builtin_ctz(Ns * R);  // N/(Ns*R) = 1 << tw_shift
+    constexpr int tw_shift = P2<N>::LOG2 - __builtin_ctz(NS * R);  // N/(NS*R) = 1 << tw_shift
+    const int k0 = t & (NS - 1);
+    const int xw = PW::idx(fb + (t - k0) * R + k0);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int j = t + TPF * q;
-        const int k = j & (Ns - 1);
+        const int k = j & (NS - 1);
         apply_twiddles<R>(v[q], tw, k << tw_shift);
         dft<R>(v[q]);
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int idx = PW::idx(fb + base + r * Ns);
+            const int idx = FOLD_WR ? xw + PW::off(TPF * q * R + r * NS) : PW::idx(fb + base + r * NS);
             re[idx] = v[q][r].x;
             im[idx] = v[q][r].y;
         }
@@ -323,41 +339,48 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         // access: it only hides the pointer's value.
         const float2* twf = tw;
         asm volatile("" : "+s"(twf));
-        // ---- pass 1: radix 8, butterflies 4t+q, inputs z[4t+q + r*N/8]
+        // ---- pass 1: radix 8, butterflies 4t+q, inputs z[4t+q + r*N/8].
+        // The offset-binary bytes enter unshifted: the radix-8 DFT reaches
+        // every output but the first through differences of its inputs, which
+        // cancel the +128 exactly (integers below 2^24), so only output 0
+        // carries 8*128*(1+i), removed after the DFT: bit-identical to
+        // subtracting 128 from every sample.
         cf v[4][8];
 #pragma unroll
         for (int r = 0; r < 8; r++) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                v[q][r] = cf{(float)((x0[r] >> (8 * q)) & 0xffu) - 128.0f,
-                             (float)((x1[r] >> (8 * q)) & 0xffu) - 128.0f};
+                v[q][r] = cf{(float)((x0[r] >> (8 * q)) & 0xffu), (float)((x1[r] >> (8 * q)) & 0xffu)};
             }
         }
         if (it + C::FPB < nave) fetch(it + C::FPB);  // next frame's bytes in flight
 #pragma unroll
-        for (int q = 0; q < 4; q++) dft8(v[q]);
+        for (int q = 0; q < 4; q++) {
+            dft8(v[q]);
+            v[q][0].x -= 1024.0f;
+            v[q][0].y -= 1024.0f;
+        }
         __syncthreads();  // previous frame's last pass has read LDS
+        const int xa = C::PadA::idx(fbase + 32 * t);  // 32t + (8q + r < 32): no carry
 #pragma unroll
         for (int q = 0; q < 4; q++) {
 #pragma unroll
             for (int r = 0; r < 8; r++) {
-                const int idx = C::PadA::idx(fbase + 32 * t + 8 * q + r);
+                const int idx = xa + C::PadA::off(8 * q + r);
                 re[idx] = v[q][r].x;
                 im[idx] = v[q][r].y;
             }
         }
         __syncthreads();
         // ---- middle passes
-        int Ns = 8;
         if constexpr (C::MID1 > 1) {
-            p2_pass<N, C::MID0, typename C::PadA, typename C::PadB>(re, im, Ns, t, fbase, twf);
-            Ns *= C::MID0;
-            p2_pass<N, C::MID1, typename C::PadB, typename C::PadC>(re, im, Ns, t, fbase, twf);
-            Ns *= C::MID1;
-        } else if constexpr (C::MID0 > 1) {
-            p2_pass<N, C::MID0, typename C::PadA, typename C::PadC>(re, im, Ns, t, fbase, twf);
-            Ns *= C::MID0;
+            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadB>(re, im, t, fbase, twf);
+            p2_pass<N, C::MID1, 8 * C::MID0, typename C::PadB, typename C::PadC>(re, im, t, fbase, twf);
+        } else {
+            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadC>(re, im, t, fbase, twf);
         }
+        // last pass reads base + r*NS: NS a multiple of the padding span folds
+        constexpr bool FOLD_LAST = NS % C::PadC::SPAN == 0;
         // ---- last pass: radix 16 on butterflies ja, jb (Ns == N/16); only
         // |Z|^2 is needed: na[r] = |Z[ja + r*NS]|^2, nb[r] = |Z[jb + r*NS]|^2
         float na[16], nb[16];
@@ -365,7 +388,8 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             cf a[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int ia = C::PadC::idx(fbase + ja + r * NS);
+                const int ia = FOLD_LAST ? C::PadC::idx(fbase + ja) + C::PadC::off(r * NS)
+                                         : C::PadC::idx(fbase + ja + r * NS);
                 a[r] = cf{re[ia], im[ia]};
             }
             apply_twiddles<16>(a, twf, ja);
@@ -377,7 +401,8 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             cf b[16];
 #pragma unroll
             for (int r = 0; r < 16; r++) {
-                const int ib = C::PadC::idx(fbase + jb + r * NS);
+                const int ib = FOLD_LAST ? C::PadC::idx(fbase + jb) + C::PadC::off(r * NS)
+                                         : C::PadC::idx(fbase + jb + r * NS);
                 b[r] = cf{re[ib], im[ib]};
             }
             apply_twiddles<16>(b, twf, jb);
