@@ -364,18 +364,32 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     }
 }
 
-// One workgroup per tile: grid = (time tiles, DM blocks).
+// Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
+// Remap the 1-D grid so that every XCD gets a contiguous range of tile ids
+// (ids 0..n-1 in XCD-major order), and order the ids DM-block fastest: the
+// workgroups an XCD runs together then share a time tile, and the window
+// rows they stage (the same rows for all DM blocks wherever the delays are
+// small) come from that XCD's L2 instead of HBM.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int bid, int n) {
+    const int q = n / kXcds, r = n % kXcds;
+    const int xcd = bid % kXcds, slot = bid / kXcds;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// One workgroup per tile: 1-D grid of nblk * (time tiles) workgroups.
 template <typename TS, int SL, int DW, int W>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
               const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
               const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
               const float* __restrict__ chan_scale, const int* __restrict__ f16_exp,
-              const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out) {
+              const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out, int nblk) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int id = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = id % nblk, tx = id / nblk;
     dedisp_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
-                               chan_scale, f16_exp, offsum, out, ld_out, blockIdx.y, blockIdx.x,
-                               lds_raw);
+                               chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -830,7 +844,10 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
                  "window holds %d rows, %d requested", p->loaded_rows, nrows);
     const TileCfg& tc = kTiles[p->tile];
     const int T = 64 * tc.sl;
-    dim3 grid((unsigned)vfrb::ceil_div(nt, T), (unsigned)p->nblk), block(64 * tc.w);
+    const int64_t nwg = vfrb::ceil_div(nt, T) * p->nblk;
+    VFRB_REQUIRE(nwg < INT32_MAX, VFRB_ERR_UNSUPPORTED, "%lld dedispersion tiles", (long long)nwg);
+    dim3 grid((unsigned)nwg), block(64 * tc.w);
+    int nblk = p->nblk;
     const void* kern = dedisp_kernel_for(p);
     VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes));
     const void* win = p->d_win;
@@ -839,7 +856,7 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
     void* args[] = {(void*)&win, (void*)&ld, (void*)&nch, (void*)&nrows, (void*)&nt,
                     (void*)&ndm, (void*)&p->d_roffs, (void*)&p->d_dmin, (void*)&p->d_chunks,
                     (void*)&p->d_blk_chunks, (void*)&p->d_u8_scale, (void*)&p->d_f16_exp,
-                    (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out};
+                    (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out, (void*)&nblk};
     VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->lds_bytes, vfrb::as_stream(stream)));
     return VFRB_OK;
 }
