@@ -18,6 +18,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -237,14 +238,19 @@ def main():
     dom = max(st, key=lambda k: st[k])
     dur_s = st[dom] * 1e-3
     if dom == "dedisperse":
+        sl, dmb, w = (int(x) for x in re.match(r"T(\d+)_dm(\d+)_w(\d+)", main_r["tile"]).groups())
+        ts = {"f32": "float", "f16": "__half", "u8": "unsigned char"}[args.storage]
+        kname = f"dedisp_kernel<{ts}, {sl // 64}, {dmb // w}, {w}>"
         ach = work[dom]["adds"] / dur_s / 1e12
         roof = dict(bound="mfma", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=ach / FP32_PEAK_TFLOPS, traffic=None, kernel="dedisp_kernel",
-                    note="no MFMA: FP32 vector-add bound; 1 add = 1 flop; adds = ndm*nch*rows_out")
+                    frac=ach / FP32_PEAK_TFLOPS, kernel=kname,
+                    note="no MFMA: FP32 vector-add bound; 1 add = 1 flop; adds = ndm*nch*rows_out per launch")
     else:
+        kname = {"filterbank": f"fb_pow2_kernel<{n}>"}.get(dom, dom)
         ach = work[dom]["bytes"] / dur_s / 1e9
         roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                    traffic=None, kernel=dom)
+                    kernel=kname)
+    roof["traffic"], roof["traffic_source"] = profiled_traffic(kname, args.storage, args.config)
     stages_roof = {}
     for k, v in st.items():
         d = v * 1e-3
@@ -311,6 +317,20 @@ def main():
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def profiled_traffic(kernel, storage, config):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a
+    calibration copy), or None when that summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "pmc_4096ch_2048dm_20s.md")
+    if config != "c2" or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        f = line.rstrip("\n").split("\t")
+        if len(f) == 4 and f[0] == "traffic_bytes_per_launch" and f[1] == storage and f[2] == kernel:
+            return float(f[3]), "profiles/pmc_4096ch_2048dm_20s.md"
+    return None, None
 
 
 def c1_shape(vfrb):
