@@ -53,3 +53,14 @@ def reference_windows(data, nfft, nave, nbundle, nskip, nch, maxnt, max_delay, o
             fb[:max_delay] = tail
             loopcnt = max_delay // nbundle
     return windows
+
+
+def reference_rows(data, nfft, nave, nbundle, nskip, nch, orc, seconds=False):
+    """Raw filterbank rows (FP64) of every bundle the reference sequencer
+    delivers, in order: the input of windows that do not overlap (STATS)."""
+    nloop = nbundle * nave * nfft
+    st, bundles = vo.reference_bundles(data, nloop, seconds=seconds)
+    assert st == 0
+    rows = [orc.filterbank_f64(np.frombuffer(a, np.uint8), np.frombuffer(b, np.uint8), nbundle, nfft, nave,
+                               nskip, nch) for a, b in bundles]
+    return np.concatenate(rows) if rows else np.zeros((0, nch))

@@ -1,6 +1,9 @@
-"""The vdifFILE2dedisp_PIPELINE program end to end on the GPU, against the
-oracle replay of the reference main loop (windows, overlap carry, the
-dumped window's filterbank and dedispersed series, SIGPROC header)."""
+"""The drop-in programs end to end on the GPU, against the oracle replay of
+the reference main loops: vdifFILE2dedisp_PIPELINE (windows, overlap carry,
+the dumped window's filterbank and dedispersed series, SIGPROC header),
+vdifDADA2timeseries (dedisperseSTATS series and sigma per window, PRESTO
+files) and vdifDADA2stats (per-channel statistics per block)."""
+import glob
 import os
 import struct
 import subprocess
@@ -8,10 +11,12 @@ import subprocess
 import numpy as np
 import pytest
 
-from cli_support import BUILD, reference_windows, write_vdif
+from cli_support import BUILD, reference_rows, reference_windows, write_vdif
 
 pytestmark = pytest.mark.gpu
 PIPE = os.path.join(BUILD, "vdifFILE2dedisp_PIPELINE")
+TS = os.path.join(BUILD, "vdifDADA2timeseries")
+ST = os.path.join(BUILD, "vdifDADA2stats")
 
 
 def test_pipeline_cli_matches_oracle(gpu, orc, tmp_path):
@@ -51,3 +56,76 @@ def test_pipeline_cli_matches_oracle(gpu, orc, tmp_path):
     assert struct.pack("<d", p.fch1) in head and struct.pack("<d", p.tchan) in head
     cands = open(tmp_path / "run.cands").read().splitlines()
     assert cands[0].startswith("#")
+
+
+def test_timeseries_cli_matches_oracle(gpu, orc, tmp_path):
+    vfrb = gpu
+    # NCHANOUT = 3840 of 4096 channels (TIMESERIES:282), a short window and a
+    # small DM list instead of 30 s and DM 119.3
+    nchan, nave, nb, window = 4096, 1, 5, 0.05
+    nfft = 2 * nchan
+    p = vfrb.params_derive(nfft, nave, nb, window, False, 3840)
+    dms = np.array([0.0, 0.3], np.float32)
+    with open(tmp_path / "dms.txt", "w") as f:
+        for d in dms:
+            f.write(f"{d:f}\n")
+    off, maxd, _ = vfrb.delay_table(dms, p.nchanout, p.fch1, p.chanbw, p.tchan)
+    ntime = p.maxnt - maxd
+    rows = p.maxnt + (p.maxnt - (maxd // nb) * nb) + nb
+    nblocks = rows * nave + 8
+    amp = vfrb.amp_for_snr(25, nave, p.nchanout)
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, nblocks, seed=6, pulses=[(0.2, 0.02, 0.0005, amp)])
+    print(f"[case] TIMESERIES CLI maxnt={p.maxnt} max_delay={maxd} ntime={ntime}", flush=True)
+    r = subprocess.run([TS, "-d", "-nchan", str(nchan), "-nave", str(nave), "-nb", str(nb), "-o", "ts",
+                        "--input-file", "in.vdif", "--skip-pairs", "0", "--window", str(window),
+                        "--dmlist", "dms.txt", "--final-inf"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    logs = "".join(open(f).read() for f in glob.glob(str(tmp_path / "ts*.log")))
+    assert r.returncode == 0, r.stderr + logs
+    data = open(tmp_path / "in.vdif", "rb").read()
+    wins = reference_windows(data, nfft, nave, nb, p.nskip, p.nchanout, p.maxnt, maxd, orc)
+    assert len(wins) >= 2
+    for k, dm in enumerate(dms):
+        dat = [f for f in glob.glob(str(tmp_path / f"DM{dm:.1f}_*.dat")) if not f.endswith("_SIGMA.dat")]
+        sig = glob.glob(str(tmp_path / f"DM{dm:.1f}_*_SIGMA.dat"))
+        inf = glob.glob(str(tmp_path / f"DM{dm:.1f}_*.inf"))
+        assert len(dat) == 1 and len(sig) == 1 and len(inf) == 1
+        series = np.fromfile(dat[0], np.float32).reshape(-1, ntime)
+        sigma = np.fromfile(sig[0], np.float32).reshape(-1, ntime)
+        assert series.shape[0] == len(wins) and sigma.shape[0] == len(wins)
+        for w, win in enumerate(wins):
+            ref = orc.dedisperse_f64(orc.bandpass_f64(win), off, ntime)[k]
+            assert np.max(np.abs(series[w] - ref)) / ref.std() < 1e-3, (k, w)
+            _, rsig = orc.dedisperse_stats_f32(orc.bandpass_f32(win.astype(np.float32)), off, ntime)
+            assert np.max(np.abs(sigma[w] / rsig[k] - 1)) < 1e-3, (k, w)
+
+
+def test_stats_cli_matches_oracle(gpu, orc, tmp_path):
+    vfrb = gpu
+    nchan, nave, nb, window, statsec = 128, 8, 5, 0.05, 0.01
+    nfft = 2 * nchan
+    p = vfrb.params_derive(nfft, nave, nb, window, fullband=True)
+    tsamp = nave * nfft / 128e6
+    # STATS:209-232: float MAXTIME/TIMESTATS; TIMESTATS (float) / tsamp (double)
+    nblk = int(np.float32(window) / np.float32(statsec))
+    ncol = int(float(np.float32(statsec)) / tsamp)
+    rows = 2 * p.maxnt + nb
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, rows * nave + 8, seed=9)
+    print(f"[case] STATS CLI maxnt={p.maxnt} blocks={nblk}x{ncol}", flush=True)
+    r = subprocess.run([ST, "--input-file", "in.vdif", "-o", "st", "--nchan", str(nchan), "--nave", str(nave),
+                        "--nb", str(nb), "--window", str(window), "--statsec", str(statsec), "--fullband"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + open(tmp_path / "st.log").read()
+    got = np.fromfile(tmp_path / "st.stats", np.float32).reshape(-1, 3, nblk, p.nchanout)
+    data = open(tmp_path / "in.vdif", "rb").read()
+    fb = reference_rows(data, nfft, nave, nb, p.nskip, p.nchanout, orc)
+    nwin = fb.shape[0] // p.maxnt
+    assert got.shape[0] == nwin >= 2
+    for w in range(nwin):
+        bp = orc.bandpass_f32(fb[w * p.maxnt:(w + 1) * p.maxnt].astype(np.float32))
+        ave, sig, out = orc.chan_stats(bp, ncol, nblk)
+        # per-block mean of bandpassed (zero-mean) data: compare on the noise scale
+        scale = np.sqrt(np.abs(sig)) + 1e-30
+        assert np.max(np.abs(got[w, 0] - ave) / scale) < 1e-3, w
+        assert np.max(np.abs(got[w, 1] / sig - 1)) < 2e-3, w
+        assert np.array_equal(got[w, 2], out), w

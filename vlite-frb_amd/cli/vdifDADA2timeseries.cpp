@@ -11,6 +11,8 @@
 //   --skip-pairs N      frame pairs discarded first (default 256000, initDADA)
 //   --rotate-windows K  start new files every K windows instead of each UTC hour
 //   --final-inf         also write the .inf headers when the input ends
+//   --window S          window length in seconds (default 30, MAXTIME)
+//   --dmlist FILE       DM trials, one per line (default 0 and 119.3)
 //   --device N
 #include <ctime>
 
@@ -27,6 +29,8 @@ struct Args {
     int rotate_windows = 0;
     bool final_inf = false;
     int device = 0;
+    double window = 30.0;     // s (MAXTIME, TIMESERIES:276-305)
+    std::string dmlist;       // one %f per line; default the reference's 0 and 119.3
 };
 
 void print_usage() {
@@ -41,7 +45,7 @@ void print_usage() {
     std::fprintf(stderr, "\t             -nchan (int):  number of frequency channels,\n");
     std::fprintf(stderr, "\t             -nb (int):     number of bundles of size (2*2*nchan*nave) to send in 1 memcpy\n");
     std::fprintf(stderr, "\t             --input-file FILE, --skip-pairs N, --rotate-windows K, --final-inf,\n");
-    std::fprintf(stderr, "\t             --device N   (MI355X build extensions)\n");
+    std::fprintf(stderr, "\t             --window S, --dmlist FILE, --device N   (MI355X build extensions)\n");
     std::fprintf(stderr, "\n  DEFAULTS:\n");
     std::fprintf(stderr, "\t             -k     %x\n", 0xdada);
     std::fprintf(stderr, "\t             -o     CRABtestDEDISP_DADA\n");
@@ -71,6 +75,8 @@ int parse(int argc, char** argv, Args& a) {
         else if (cli::eq(s, "--rotate-windows")) a.rotate_windows = std::atoi(next());
         else if (cli::eq(s, "--final-inf")) a.final_inf = true;
         else if (cli::eq(s, "--device")) a.device = std::atoi(next());
+        else if (cli::eq(s, "--window")) a.window = std::atof(next());
+        else if (cli::eq(s, "--dmlist")) a.dmlist = next();
         else {
             std::fprintf(stderr, "Error! Unknown argument: %s\n\n", s);
             print_usage();
@@ -95,9 +101,10 @@ int utc_hour() {
 }
 
 struct SeriesFiles {
-    std::string base[2];
-    void open_new(const float* dms, const std::string& stamp) {
-        for (int k = 0; k < 2; k++) {
+    std::vector<std::string> base;
+    void open_new(const std::vector<float>& dms, const std::string& stamp) {
+        base.resize(dms.size());
+        for (size_t k = 0; k < dms.size(); k++) {
             char b[256];
             std::snprintf(b, sizeof b, "DM%.1f_%s", dms[k], stamp.c_str());
             base[k] = b;
@@ -155,7 +162,7 @@ int main(int argc, char** argv) {
     std::fprintf(flog, "Running on device: %s\n", devname);
 
     vfrb_params p;
-    CLI_CHECK(vfrb_params_derive(a.twonchan, a.nave, a.nbundle, 30.0, 0, 3840, &p), 1);
+    CLI_CHECK(vfrb_params_derive(a.twonchan, a.nave, a.nbundle, a.window, 0, 3840, &p), 1);
     const int maxloopcnt = p.maxnt / p.nbundle;
     std::fprintf(flog, "NFREQ= %d\nNCHAN= %d\nNCHANOUT= %d\nNAVE= %d\nNBUNDLE= %d\n", p.nfreq, p.nchan, p.nchanout,
                  p.nave, p.nbundle);
@@ -184,7 +191,23 @@ int main(int argc, char** argv) {
         return -1;
     }
 
-    const float dms[2] = {0.0f, 119.3f};
+    std::vector<float> dms = {0.0f, 119.3f};  // TIMESERIES:441-442
+    if (!a.dmlist.empty()) {
+        dms.clear();
+        FILE* fd = std::fopen(a.dmlist.c_str(), "r");
+        if (!fd) {
+            std::fprintf(stderr, "Unable to open DM list: %s\n", a.dmlist.c_str());
+            return 2;
+        }
+        float v;
+        while (std::fscanf(fd, "%f", &v) == 1) dms.push_back(v);
+        std::fclose(fd);
+        if (dms.empty()) {
+            std::fprintf(stderr, "Error! empty DM list %s\n", a.dmlist.c_str());
+            return 1;
+        }
+    }
+    const int ndm = (int)dms.size();
     vfrb_fb_plan* fbp = nullptr;
     if (vfrb_fb_plan_create(&fbp, p.twonchan, p.nave, p.nskip, p.nchanout) != VFRB_OK) {
         std::fprintf(flog, "Error creating FFT plan!\n%s\n", vfrb_last_error());
@@ -195,8 +218,8 @@ int main(int argc, char** argv) {
     sd.tsamp = p.tchan;
     sd.fch1 = p.fch1;
     sd.chanbw = p.chanbw;
-    sd.ndm = 2;
-    sd.dms = dms;
+    sd.ndm = ndm;
+    sd.dms = dms.data();
     sd.max_rows = p.maxnt;
     sd.storage = VFRB_STORE_F32;
     sd.tile_config = -1;
@@ -204,7 +227,7 @@ int main(int argc, char** argv) {
     CLI_CHECK(vfrb_search_plan_create(&sp, &sd), 1);
     const int max_delay = vfrb_search_max_delay(sp);
     const int ntime = p.maxnt - max_delay;
-    std::fprintf(flog, "dm_count= %d  max_delay= %d\n", 2, max_delay);
+    std::fprintf(flog, "dm_count= %d  max_delay= %d\n", ndm, max_delay);
     if (ntime <= 0) {
         std::fprintf(flog, "ERROR: window of %d rows is not longer than max_delay %d\n", p.maxnt, max_delay);
         return 1;
@@ -223,9 +246,9 @@ int main(int argc, char** argv) {
     CLI_CHECK(vfrb_malloc((void**)&dfb, fb_bytes), 2);
     CLI_CHECK(vfrb_memset(dfb, 0, fb_bytes, nullptr), 2);
     CLI_CHECK(vfrb_malloc((void**)&dtail, tail_bytes ? tail_bytes : 4), 2);
-    CLI_CHECK(vfrb_malloc((void**)&dout, 2 * ntime * sizeof(float)), 2);
-    CLI_CHECK(vfrb_malloc((void**)&dsig, 2 * ntime * sizeof(float)), 2);
-    std::vector<float> hout(2 * (size_t)ntime), hsig(2 * (size_t)ntime);
+    CLI_CHECK(vfrb_malloc((void**)&dout, (size_t)ndm * ntime * sizeof(float)), 2);
+    CLI_CHECK(vfrb_malloc((void**)&dsig, (size_t)ndm * ntime * sizeof(float)), 2);
+    std::vector<float> hout((size_t)ndm * ntime), hsig((size_t)ndm * ntime);
 
     std::fprintf(flog, "initializing pol 0 & 1 single frame buffers...\n");
     vfrb_status ist = vfrb_seq_init(seq);
@@ -269,7 +292,7 @@ int main(int argc, char** argv) {
         CLI_CHECK(vfrb_search_dedisperse_stats(sp, p.maxnt, dout, dsig, ntime, nullptr), 1);
         CLI_CHECK(vfrb_memcpy_d2h(hout.data(), dout, hout.size() * sizeof(float), nullptr), 2);
         CLI_CHECK(vfrb_memcpy_d2h(hsig.data(), dsig, hsig.size() * sizeof(float), nullptr), 2);
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < ndm; k++) {
             files.append(k, ".dat", hout.data() + (size_t)k * ntime, ntime);
             files.append(k, "_SIGMA.dat", hsig.data() + (size_t)k * ntime, ntime);
         }
@@ -279,7 +302,7 @@ int main(int argc, char** argv) {
         q++;
         const bool rollover = a.rotate_windows > 0 ? (q % a.rotate_windows == 0) : (utc_hour() == hrnext);
         if (rollover) {
-            for (int k = 0; k < 2; k++)
+            for (int k = 0; k < ndm; k++)
                 write_inf(files.base[k], mjdstart, ntimes, tsamp, dms[k], fchlo, bandw, p.nchanout, p.chanbw);
             stamp = utc_stamp();
             if (a.rotate_windows > 0) stamp += "_w" + std::to_string(q);
@@ -301,7 +324,7 @@ int main(int argc, char** argv) {
         std::fflush(flog);
     }
     if (a.final_inf)
-        for (int k = 0; k < 2; k++)
+        for (int k = 0; k < ndm; k++)
             write_inf(files.base[k], mjdstart, ntimes, tsamp, dms[k], fchlo, bandw, p.nchanout, p.chanbw);
     vfrb_seq_close(seq);
     vfrb_fb_plan_destroy(fbp);
