@@ -152,8 +152,8 @@ def main():
     maxc = 1 << 20
     cdm, ct = vfrb.DeviceArray(maxc, np.int32), vfrb.DeviceArray(maxc, np.int32)
     csnr, ccount = vfrb.DeviceArray(maxc, np.float32), vfrb.DeviceArray(1, np.int32)
-    stream = vfrb.Stream()
-    s = stream.h
+    stream, stream2 = vfrb.Stream(), vfrb.Stream()
+    s, s2 = stream.h, stream2.h
 
     storages = [args.storage] + [x for x in args.compare.split(",") if x and x != args.storage]
     results = {}
@@ -168,55 +168,85 @@ def main():
             tune = sp.autotune(win_rows, out, seg_rows, iters=2, stream=s)
             log(f"{sname}: autotuned tile {vfrb.tile_names()[sp.tile]}: " +
                 " ".join(f"{k}={v:.2f}" for k, v in tune.items()))
-        ev = [vfrb.Event() for _ in range(6)]
         names = ["filterbank", "bandpass_pack", "dedisperse", "dm_stats", "candidates"]
-        stage_ms = {k: [] for k in names}
-        step_ms = []
 
-        def step(i, timed):
-            new0 = (max_delay + i * seg_rows) % ring_rows
-            win0 = (i * seg_rows) % ring_rows
-            ev[0].record(s)
-            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=new0, stream=s)
-            ev[1].record(s)
-            sp.load(ring, win_rows, ring_rows=ring_rows, row0=win0, bandpass=True, stream=s)
-            ev[2].record(s)
-            sp.dedisperse(win_rows, out, seg_rows, stream=s)
-            ev[3].record(s)
+        def front(i, st):
+            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=(max_delay + i * seg_rows) % ring_rows,
+                        stream=st)
+
+        def back(i, st, marks=None):
+            sp.load(ring, win_rows, ring_rows=ring_rows, row0=(i * seg_rows) % ring_rows, bandpass=True, stream=st)
+            marks and marks[0].record(st)
+            sp.dedisperse(win_rows, out, seg_rows, stream=st)
+            marks and marks[1].record(st)
             vfrb.call("vfrb_dm_stats", out.ptr, seg_rows, ndm, seg_rows, acc, frb.ptr, ave.ptr, sig.ptr,
-                      peak.ptr, peak_t.ptr, s)
-            ev[4].record(s)
+                      peak.ptr, peak_t.ptr, st)
+            marks and marks[2].record(st)
             vfrb.call("vfrb_candidates", out.ptr, seg_rows, ndm, seg_rows, ave.ptr, sig.ptr, 7.0, cdm.ptr,
-                      ct.ptr, csnr.ptr, maxc, ccount.ptr, s)
+                      ct.ptr, csnr.ptr, maxc, ccount.ptr, st)
+
+        # (1) serial steps on one stream: per-kernel times, free of overlap
+        ev = [vfrb.Event() for _ in range(6)]
+        stage_ms = {k: [] for k in names}
+        nserial = max(2, min(args.steps, 4))
+        for i in range(args.warmup + nserial):
+            ev[0].record(s)
+            front(i, s)
+            ev[1].record(s)
+            back(i, s, ev[2:5])
             ev[5].record(s)
-            if timed:
+            if i >= args.warmup:
                 ev[5].sync()
                 for k, nm in enumerate(names):
                     stage_ms[nm].append(ev[k].elapsed_ms(ev[k + 1]))
-                step_ms.append(ev[0].elapsed_ms(ev[5]))
-
-        for i in range(args.warmup):
-            step(i, False)
         stream.sync()
-        vfrb.sync()
-        t0 = time.perf_counter()
-        for i in range(args.steps):
-            step(args.warmup + i, True)
+        # (2) the timed run: segments pipelined over two streams.  The front
+        # stream filterbanks segment i+1 into the ring while the back stream
+        # dedisperses segment i; the front waits until the back has loaded
+        # window i (whose rows segment i+1 overwrites), the back waits for the
+        # front's rows.  Latency of a step = its filterbank start to its
+        # candidates.
+        nst = args.warmup + args.steps
+        ev_s = [vfrb.Event() for _ in range(nst)]
+        ev_f = [vfrb.Event() for _ in range(nst)]
+        ev_l = [vfrb.Event() for _ in range(nst)]
+        ev_e = [vfrb.Event() for _ in range(nst)]
+        spare = [vfrb.Event(), vfrb.Event()]
+        t0 = None
+        for i in range(nst):
+            if i == args.warmup:
+                stream.sync()
+                stream2.sync()
+                vfrb.sync()
+                t0 = time.perf_counter()
+            if i > 0:
+                stream2.wait(ev_l[i - 1])
+            ev_s[i].record(s2)
+            front(args.warmup + nserial + i, s2)
+            ev_f[i].record(s2)
+            stream.wait(ev_f[i])
+            back(args.warmup + nserial + i, s, [ev_l[i]] + spare)
+            ev_e[i].record(s)
         stream.sync()
+        stream2.sync()
         vfrb.sync()
         wall = time.perf_counter() - t0
+        step_ms = [ev_s[i].elapsed_ms(ev_e[i]) for i in range(args.warmup, nst)]
         ms_step = wall * 1e3 / args.steps
         rtf = args.steps * seg_data_s / wall
+        serial_ms = float(np.sum([np.mean(v) for v in stage_ms.values()]))
         ncand = int(ccount.download()[0])
         pk = peak.download()
         best = int(np.argmax(pk))
         lat = np.array(step_ms)
         results[sname] = dict(rtf=rtf, ms_per_step=ms_step, wall_s=wall,
                               stage_ms={k: float(np.mean(v)) for k, v in stage_ms.items()},
+                              serial_ms_per_step=serial_ms, serial_rtf=seg_data_s / (serial_ms * 1e-3),
                               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                               candidates=ncand, best_dm=float(dms[best]), best_snr=float(pk[best]),
                               tile=vfrb.tile_names()[sp.tile], autotune_ms=tune)
-        log(f"{sname}: RTF={rtf:.1f} ms/step={ms_step:.2f} stages="
+        log(f"{sname}: RTF={rtf:.1f} (serial {seg_data_s / (serial_ms * 1e-3):.1f}) ms/step={ms_step:.2f} "
+            f"p99={np.percentile(lat, 99):.2f} stages="
             + " ".join(f"{k}={np.mean(v):.3f}" for k, v in stage_ms.items())
             + f" cand={ncand} best DM={dms[best]:.1f} SNR={pk[best]:.1f}")
         del sp
@@ -300,11 +330,13 @@ def main():
                    "segment_rows": seg_rows, "max_delay_rows": max_delay, "window_rows": win_rows,
                    "storage": args.storage, "accumulate": "f32",
                    "input": "VDIF payload resident in HBM (device-resident RTF)",
+                   "schedule": "segments pipelined on two HIP streams (filterbank of i+1 beside dedispersion of i)",
                    "parallelism": "single GPU"},
         "p50_latency_ms": main_r["p50_ms"],
         "p99_latency_ms": main_r["p99_ms"],
         "segment_ms": seg_data_s * 1e3,
         "rtf_by_storage": {k: v["rtf"] for k, v in results.items()},
+        "serial_rtf_by_storage": {k: v["serial_rtf"] for k, v in results.items()},
         "p99_ms_by_storage": {k: v["p99_ms"] for k, v in results.items()},
         "stage_ms_by_storage": {k: v["stage_ms"] for k, v in results.items()},
         "candidates_by_storage": {k: v["candidates"] for k, v in results.items()},

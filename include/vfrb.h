@@ -132,6 +132,8 @@ vfrb_status vfrb_event_destroy(void* event);
 vfrb_status vfrb_event_record(void* event, void* stream);
 vfrb_status vfrb_event_synchronize(void* event);
 vfrb_status vfrb_event_elapsed_ms(float* ms, void* start, void* stop);
+/* Work queued on `stream` after this call waits for `event` (cross-stream order). */
+vfrb_status vfrb_stream_wait_event(void* stream, void* event);
 
 /* ------------------------------------------------------------------------ */
 /* Front end: fused unpack + R2C FFT + detection + pol sum + time average   */

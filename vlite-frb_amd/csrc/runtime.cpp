@@ -181,6 +181,12 @@ vfrb_status vfrb_event_synchronize(void* event) {
     return VFRB_OK;
 }
 
+vfrb_status vfrb_stream_wait_event(void* stream, void* event) {
+    VFRB_REQUIRE(event, VFRB_ERR_INVALID, "event is NULL");
+    VFRB_HIP(hipStreamWaitEvent(vfrb::as_stream(stream), reinterpret_cast<hipEvent_t>(event), 0));
+    return VFRB_OK;
+}
+
 vfrb_status vfrb_event_elapsed_ms(float* ms, void* start, void* stop) {
     VFRB_REQUIRE(ms, VFRB_ERR_INVALID, "ms is NULL");
     VFRB_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),

@@ -123,6 +123,7 @@ _SIGS = {
     "vfrb_event_record": (i32, [vp, vp]),
     "vfrb_event_synchronize": (i32, [vp]),
     "vfrb_event_elapsed_ms": (i32, [P(f32), vp, vp]),
+    "vfrb_stream_wait_event": (i32, [vp, vp]),
     "vfrb_fb_plan_create": (i32, [P(vp), i32, i32, i32, i32]),
     "vfrb_fb_plan_destroy": (i32, [vp]),
     "vfrb_fb_execute": (i32, [vp, vp, vp, i64, vp, i64, i64, vp]),
@@ -403,6 +404,9 @@ class Stream:
 
     def sync(self):
         call("vfrb_stream_synchronize", self.h)
+
+    def wait(self, event: "Event"):
+        call("vfrb_stream_wait_event", self.h, event.h)
 
     def __del__(self):
         try:
