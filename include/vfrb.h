@@ -162,6 +162,16 @@ vfrb_status vfrb_fb_plan_destroy(vfrb_fb_plan* plan);
 vfrb_status vfrb_fb_execute(vfrb_fb_plan* plan, const uint8_t* pol0,
                             const uint8_t* pol1, int64_t nrows, float* fb,
                             int64_t ring_rows, int64_t row0, void* stream);
+/*
+ * The same filterbank read straight from raw frame pairs through an ingest
+ * slot table (power-of-two FFT lengths): row r's bytes are payload-stream
+ * bytes byte0 + r*nave*twonchan onwards (byte0 a multiple of 4), zero slots
+ * reading as zero bytes.  No de-framed copy is made.
+ */
+vfrb_status vfrb_fb_execute_frames(vfrb_fb_plan* plan, const uint8_t* frames,
+                                   const int32_t* table, int64_t nslots, int64_t byte0,
+                                   int64_t nrows, float* fb, int64_t ring_rows,
+                                   int64_t row0, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Bandpass (time-major, in place)                                          */
@@ -183,6 +193,37 @@ vfrb_status vfrb_bandpass_inplace(float* fb, int32_t nrows, int32_t nch, void* s
 vfrb_status vfrb_chan_stats(const float* fb, int32_t nrows, int32_t nch,
                             int32_t ncolstatblk, int32_t nstatblks,
                             float* ave, float* sig, float* outlier, void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Device-side frame ingest (no reference counterpart: the reference copies   */
+/* payloads frame by frame on the host, PIPELINE:934-1171)                    */
+/* ------------------------------------------------------------------------ */
+/*
+ * Raw VDIF frame pairs already in device memory (a NIC-to-GPU path): pair k
+ * is frames 2k (thread 0 = pol 0) and 2k+1 (thread 1 = pol 1), 5032 bytes
+ * each, the first pair being the one the reference's initframe01 +
+ * initskip01 start on (thread-aligned, cal boundary).  vfrb_ingest_frames
+ * validates every pair and builds the slot table of the two payload streams
+ * with the reference's zero-filled runs for missing frames (checkframeskip,
+ * PIPELINE:888-930; seconds-aware rule TIMESERIES:893-937 when
+ * seconds_check): table[s] = pair feeding payload slot s (5000 bytes per pol)
+ * or -1 for a zero slot.  The parse state (frame number expected next)
+ * carries from one call to the next, so a stream may arrive in batches, each
+ * with its own table starting at slot 0.  A pair whose threads are not 0/1
+ * or whose frame numbers (seconds) differ is VFRB_ERR_FORMAT: the host
+ * sequencer reproduces the reference on such streams.  Synchronizes `stream`
+ * (the slot count is returned to the host).
+ */
+typedef struct vfrb_ingest vfrb_ingest;
+vfrb_status vfrb_ingest_create(vfrb_ingest** ingest, int64_t max_pairs, int32_t seconds_check);
+vfrb_status vfrb_ingest_destroy(vfrb_ingest* ingest);
+/* Expected next frame number and second; -1 = accept the next pair as is. */
+vfrb_status vfrb_ingest_reset(vfrb_ingest* ingest, int32_t expected_frame, int32_t expected_second);
+vfrb_status vfrb_ingest_frames(vfrb_ingest* ingest, const uint8_t* frames, int64_t npairs,
+                               int32_t* table, int64_t table_cap, int64_t* nslots, void* stream);
+/* The de-framed payload of one pol: nslots*5000 bytes (zero slots as zeros). */
+vfrb_status vfrb_ingest_gather(const uint8_t* frames, const int32_t* table, int64_t nslots,
+                               int32_t pol, uint8_t* out, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Back end: search plan (replaces the dedisp plan)                         */

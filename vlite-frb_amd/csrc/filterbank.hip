@@ -287,11 +287,18 @@ __device__ __forceinline__ float norm2(cf a) { return a.x * a.x + a.y * a.y; }
 
 // LDS (70 KB for N <= 8192) allows two workgroups per CU, i.e. 2 waves per
 // SIMD: hold the register allocator to that budget.
-template <int N>
+// Raw VDIF frames (FRAMED): pol0 points at frame pairs (thread 0 then 1,
+// 5032 bytes each), table[s] is the pair feeding payload slot s (5000 bytes
+// per pol) or -1 for a zero-filled slot, and byte0 is the payload-stream
+// offset of row 0.  Otherwise pol0/pol1 are the de-framed payload streams.
+constexpr int kFrameBytes = 5032, kHeaderBytes = 32, kPayloadBytes = 5000;
+
+template <int N, bool FRAMED>
 __global__ void __launch_bounds__(P2<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
 fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                const float2* __restrict__ tw, float* __restrict__ fb, int64_t nrows, int nave,
-               int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale) {
+               int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale,
+               const int32_t* __restrict__ table, int64_t byte0) {
     using C = P2<N>;
     constexpr int TPF = C::TPF;
     constexpr int NS = N / 16;      // butterflies of the last pass
@@ -318,7 +325,36 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
     uint32_t x0[8], x1[8];
     auto fetch = [&](int it) {
         const int frame = it + slot;
-        if (frame < nave) {
+        if (frame < nave && FRAMED) {
+            // payload offset -> (slot, byte): one FP64 division, then steps of
+            // N/8 < 5000 bytes carry into the next slot at most once
+            const int64_t o = byte0 + ((int64_t)row * nave + frame) * N + 4 * t;
+            int64_t sl = (int64_t)((double)o * (1.0 / kPayloadBytes));
+            int w = (int)(o - sl * kPayloadBytes);
+            if (w < 0) {
+                sl--;
+                w += kPayloadBytes;
+            } else if (w >= kPayloadBytes) {
+                sl++;
+                w -= kPayloadBytes;
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int k = table[sl];
+                if (k >= 0) {
+                    const uint8_t* f = pol0 + (int64_t)k * (2 * kFrameBytes) + kHeaderBytes + w;
+                    x0[r] = *reinterpret_cast<const uint32_t*>(f);
+                    x1[r] = *reinterpret_cast<const uint32_t*>(f + kFrameBytes);
+                } else {
+                    x0[r] = x1[r] = 0u;  // zero-filled payload of a missing frame
+                }
+                w += EIGHTH;
+                if (w >= kPayloadBytes) {
+                    w -= kPayloadBytes;
+                    sl++;
+                }
+            }
+        } else if (frame < nave) {
             const uint8_t* b0 = pol0 + ((size_t)row * nave + frame) * N + 4 * t;
             const uint8_t* b1 = pol1 + ((size_t)row * nave + frame) * N + 4 * t;
 #pragma unroll
@@ -579,13 +615,15 @@ struct vfrb_fb_plan {
     size_t lds_bytes = 0;
     int block = 256;
     const void* kernel = nullptr;
+    const void* kernel_framed = nullptr;  // power-of-two lengths only
 };
 
 namespace {
 
 template <int N>
 void select_pow2(vfrb_fb_plan* p) {
-    p->kernel = reinterpret_cast<const void*>(&fb_pow2_kernel<N>);
+    p->kernel = reinterpret_cast<const void*>(&fb_pow2_kernel<N, false>);
+    p->kernel_framed = reinterpret_cast<const void*>(&fb_pow2_kernel<N, true>);
     p->lds_bytes = 2 * sizeof(float) * P2<N>::LDS_WORDS;
     p->block = P2<N>::BLOCK;
 }
@@ -654,6 +692,8 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
     if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
     if (e == hipSuccess && p->lds_bytes > 65536)
         e = hipFuncSetAttribute(p->kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes);
+    if (e == hipSuccess && p->kernel_framed && p->lds_bytes > 65536)
+        e = hipFuncSetAttribute(p->kernel_framed, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes);
     if (e != hipSuccess) {
         if (p->d_tw) (void)hipFree(p->d_tw);
         delete p;
@@ -686,9 +726,11 @@ vfrb_status vfrb_fb_execute(vfrb_fb_plan* p, const uint8_t* pol0, const uint8_t*
     hipStream_t s = vfrb::as_stream(stream);
     dim3 grid((unsigned)nrows), block(p->block);
     if (p->pow2) {
+        const int32_t* no_table = nullptr;
+        int64_t no_byte0 = 0;
         void* args[] = {(void*)&pol0, (void*)&pol1, (void*)&p->d_tw, (void*)&fb,  (void*)&nrows,
                         (void*)&p->nave, (void*)&p->nskip, (void*)&p->nchanout, (void*)&ring_rows,
-                        (void*)&row0, (void*)&scale};
+                        (void*)&row0, (void*)&scale, (void*)&no_table, (void*)&no_byte0};
         VFRB_HIP(hipLaunchKernel(p->kernel, grid, block, args, p->lds_bytes, s));
     } else {
         hipLaunchKernelGGL(fb_generic_kernel, grid, block, p->lds_bytes, s, pol0, pol1,
@@ -696,6 +738,30 @@ vfrb_status vfrb_fb_execute(vfrb_fb_plan* p, const uint8_t* pol0, const uint8_t*
                            p->nchanout, ring_rows, row0, scale);
         VFRB_LAUNCH_CHECK("fb_generic_kernel");
     }
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_fb_execute_frames(vfrb_fb_plan* p, const uint8_t* frames, const int32_t* table,
+                                   int64_t nslots, int64_t byte0, int64_t nrows, float* fb,
+                                   int64_t ring_rows, int64_t row0, void* stream) {
+    VFRB_REQUIRE(p && frames && table && fb, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(p->pow2, VFRB_ERR_UNSUPPORTED, "frame input needs a power-of-two FFT length");
+    VFRB_REQUIRE(nrows >= 0 && row0 >= 0 && ring_rows >= 0 && byte0 >= 0 && nslots >= 0,
+                 VFRB_ERR_INVALID, "negative size");
+    VFRB_REQUIRE(byte0 % 4 == 0 && ((uintptr_t)frames % 4) == 0, VFRB_ERR_INVALID,
+                 "frames and byte0 must be 4-byte aligned");
+    VFRB_REQUIRE(ring_rows == 0 || nrows <= ring_rows, VFRB_ERR_INVALID, "nrows exceeds ring");
+    VFRB_REQUIRE(nrows <= INT32_MAX, VFRB_ERR_INVALID, "nrows too large for one launch");
+    VFRB_REQUIRE(byte0 + nrows * (int64_t)p->nave * p->N <= nslots * (int64_t)kPayloadBytes,
+                 VFRB_ERR_INVALID, "rows reach past the %lld table slots", (long long)nslots);
+    if (nrows == 0) return VFRB_OK;
+    const float scale = (float)(0.5 / ((double)(p->N / 2 + 1) * (double)p->nave));
+    const uint8_t* none = nullptr;
+    void* args[] = {(void*)&frames, (void*)&none, (void*)&p->d_tw, (void*)&fb,  (void*)&nrows,
+                    (void*)&p->nave, (void*)&p->nskip, (void*)&p->nchanout, (void*)&ring_rows,
+                    (void*)&row0, (void*)&scale, (void*)&table, (void*)&byte0};
+    VFRB_HIP(hipLaunchKernel(p->kernel_framed, dim3((unsigned)nrows), dim3(p->block), args, p->lds_bytes,
+                             vfrb::as_stream(stream)));
     return VFRB_OK;
 }
 

@@ -127,6 +127,12 @@ _SIGS = {
     "vfrb_fb_plan_create": (i32, [P(vp), i32, i32, i32, i32]),
     "vfrb_fb_plan_destroy": (i32, [vp]),
     "vfrb_fb_execute": (i32, [vp, vp, vp, i64, vp, i64, i64, vp]),
+    "vfrb_fb_execute_frames": (i32, [vp, vp, vp, i64, i64, i64, vp, i64, i64, vp]),
+    "vfrb_ingest_create": (i32, [P(vp), i64, i32]),
+    "vfrb_ingest_destroy": (i32, [vp]),
+    "vfrb_ingest_reset": (i32, [vp, i32, i32]),
+    "vfrb_ingest_frames": (i32, [vp, vp, i64, vp, i64, P(i64), vp]),
+    "vfrb_ingest_gather": (i32, [vp, vp, i64, i32, vp, vp]),
     "vfrb_bandpass_inplace": (i32, [vp, i32, i32, vp]),
     "vfrb_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "vfrb_search_plan_create": (i32, [P(vp), P(SearchDesc)]),
@@ -459,10 +465,47 @@ class FilterbankPlan:
         call("vfrb_fb_execute", self.h, pol0.offset(pol_offset), pol1.offset(pol_offset), nrows,
              fb.ptr, ring_rows, row0, _s(stream))
 
+    def execute_frames(self, frames: DeviceArray, table: DeviceArray, nslots, byte0, nrows, fb: DeviceArray,
+                       ring_rows=0, row0=0, stream=None):
+        """Filterbank straight from raw frame pairs through an ingest slot table."""
+        call("vfrb_fb_execute_frames", self.h, frames.ptr, table.ptr, nslots, byte0, nrows, fb.ptr, ring_rows,
+             row0, _s(stream))
+
     def __del__(self):
         try:
             if self.h:
                 lib.vfrb_fb_plan_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Ingest:
+    """Device-side VDIF frame ingest (vfrb_ingest_*): raw frame pairs in
+    device memory -> payload slot table with the reference's zero fill."""
+
+    def __init__(self, max_pairs, seconds_check=False):
+        self.h = vp()
+        call("vfrb_ingest_create", C.byref(self.h), max_pairs, int(bool(seconds_check)))
+
+    def reset(self, expected_frame=-1, expected_second=0):
+        call("vfrb_ingest_reset", self.h, expected_frame, expected_second)
+
+    def frames(self, frames: DeviceArray, npairs, table: DeviceArray, stream=None, offset=0):
+        """Builds the table of npairs pairs starting `offset` bytes into
+        `frames`; returns the payload slots they produce."""
+        n = i64(0)
+        call("vfrb_ingest_frames", self.h, frames.offset(offset), npairs, table.ptr, table.nbytes // 4,
+             C.byref(n), _s(stream))
+        return n.value
+
+    @staticmethod
+    def gather(frames: DeviceArray, table: DeviceArray, nslots, pol, out: DeviceArray, stream=None, offset=0):
+        call("vfrb_ingest_gather", frames.offset(offset), table.ptr, nslots, pol, out.ptr, _s(stream))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_ingest_destroy(self.h)
         except Exception:
             pass
 
