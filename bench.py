@@ -62,6 +62,8 @@ def main():
                     help="file to reuse the synthesised payload between runs (written if absent)")
     ap.add_argument("--sections", default="precision,batching,serving",
                     help="extra measurements reported in the same line (empty = none)")
+    ap.add_argument("--input", default="frames", choices=["frames", "payload"],
+                    help="device-resident input: raw VDIF frame pairs (ingested on the GPU) or de-framed payload")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="wall seconds of each replayed arrival trace")
     args = ap.parse_args()
@@ -109,16 +111,21 @@ def main():
     tone_amp = 2.0 * 8.0 * np.sqrt(5.0 / (n * np.sqrt(2.0 * nave)))
     tones = [(float(rng.integers(1, n // 2)), tone_amp) for _ in range(W["ntone"])]
     spec = vfrb.synth_spec(n, seed=W["seed"], pulses=pulses, tones=tones)
-    nblk = seg_rows * nave
+    frames_mode = args.input == "frames"
+    npairs = -(-seg_rows * nave * n // 5000)      # VDIF frames per pol covering one segment
+    nblk = -(-npairs * 5000 // n)                 # synthesised FFT blocks (covers both layouts)
     t = time.perf_counter()
-    d0 = vfrb.DeviceArray(nblk * n, np.uint8)
-    d1 = vfrb.DeviceArray(nblk * n, np.uint8)
     chunk = max(1, (1 << 28) // n)  # 256 MB host chunks
-    host_payload = {} if args.no_cpu_baseline else {0: np.empty(nblk * n, np.uint8), 1: np.empty(nblk * n, np.uint8)}
+    keep = frames_mode or not args.no_cpu_baseline
+    host_payload = {0: np.empty(nblk * n, np.uint8), 1: np.empty(nblk * n, np.uint8)} if keep else {}
+    d0 = d1 = raw = None
+    if not frames_mode:
+        d0 = vfrb.DeviceArray(nblk * n, np.uint8)
+        d1 = vfrb.DeviceArray(nblk * n, np.uint8)
     cache = args.payload_cache
     reuse = bool(cache) and os.path.exists(cache) and os.path.getsize(cache) == 2 * nblk * n
     fcache = open(cache, "rb" if reuse else "wb") if cache else None
-    for pol, d in ((0, d0), (1, d1)):
+    for pol in (0, 1):
         for b0 in range(0, nblk, chunk):
             nb = min(chunk, nblk - b0)
             if reuse:
@@ -129,14 +136,38 @@ def main():
                     h.tofile(fcache)
             if host_payload:
                 host_payload[pol][b0 * n: (b0 + nb) * n] = h
-            vfrb.call("vfrb_memcpy_h2d", d.offset(b0 * n), h.ctypes.data_as(C.c_void_p), h.nbytes, None)
+            if not frames_mode:
+                d = d0 if pol == 0 else d1
+                vfrb.call("vfrb_memcpy_h2d", d.offset(b0 * n), h.ctypes.data_as(C.c_void_p), h.nbytes, None)
     if fcache:
         fcache.close()
-    log(f"{'loaded' if reuse else 'synthesised'} + uploaded {2 * nblk * n / 1e9:.2f} GB in "
-        f"{time.perf_counter() - t:.1f} s")
+    if frames_mode:
+        # VDIF frame pairs as the instrument sends them: 32-byte headers
+        # (seconds from 1000, frame 0..25599, thread 0 then 1) + 5000 bytes
+        raw = vfrb.DeviceArray(npairs * 2 * 5032, np.uint8)
+        step = 65536
+        for k0 in range(0, npairs, step):
+            k = np.arange(k0, min(npairs, k0 + step))
+            words = np.zeros((k.size, 8), np.uint32)
+            words[:, 0] = 1000 + k // 25600
+            words[:, 1] = (k % 25600) | (30 << 24)
+            words[:, 2] = 629
+            words[:, 3] = 0x5856 | (7 << 26)
+            blk = np.empty((k.size, 2, 5032), np.uint8)
+            blk[:, 0, :32] = words.view(np.uint8).reshape(k.size, 32)
+            words[:, 3] |= 1 << 16
+            blk[:, 1, :32] = words.view(np.uint8).reshape(k.size, 32)
+            for pol in (0, 1):
+                blk[:, pol, 32:] = host_payload[pol][k[0] * 5000:(k[-1] + 1) * 5000].reshape(k.size, 5000)
+            vfrb.call("vfrb_memcpy_h2d", raw.offset(k0 * 2 * 5032), blk.ctypes.data_as(C.c_void_p), blk.nbytes,
+                      None)
+        ing = vfrb.Ingest(npairs)
+        table = vfrb.DeviceArray(npairs + 4096, np.int32)
+    log(f"{'loaded' if reuse else 'synthesised'} + uploaded {2 * nblk * n / 1e9:.2f} GB "
+        f"({'VDIF frame pairs' if frames_mode else 'payload'}) in {time.perf_counter() - t:.1f} s")
     # plain copy of known size: the calibration kernel for the traffic counters
     calib = vfrb.DeviceArray(1 << 28, np.uint8)
-    vfrb.call("vfrb_memcpy_d2d", calib.ptr, d0.ptr, 1 << 28, None)
+    vfrb.call("vfrb_memcpy_d2d", calib.ptr, (raw if frames_mode else d0).ptr, 1 << 28, None)
     vfrb.sync()
     del calib
 
@@ -155,6 +186,24 @@ def main():
     stream, stream2 = vfrb.Stream(), vfrb.Stream()
     s, s2 = stream.h, stream2.h
 
+    names = (["ingest"] if frames_mode else []) + ["filterbank", "bandpass_pack", "dedisperse", "dm_stats",
+                                                      "candidates"]
+
+    def front(i, st, mark=None):
+        """Segment i's new rows into the ring: from the raw frame pairs
+        through the device ingest's slot table, or from the payload."""
+        row0 = (max_delay + i * seg_rows) % ring_rows
+        if frames_mode:
+            # every step replays the same frames: restart the parse state
+            # (a live stream carries it from one batch to the next)
+            ing.reset()
+            nsl = ing.frames(raw, npairs, table, stream=st)
+            if mark is not None:
+                mark.record(st)
+            fbp.execute_frames(raw, table, nsl, 0, seg_rows, ring, ring_rows=ring_rows, row0=row0, stream=st)
+        else:
+            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=row0, stream=st)
+
     storages = [args.storage] + [x for x in args.compare.split(",") if x and x != args.storage]
     results = {}
     for sname in storages:
@@ -163,16 +212,11 @@ def main():
         tune = None
         if args.tile < 0:
             # autotune the dedispersion tile on a real window (not timed)
-            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=max_delay, stream=s)
+            front(0, s)
             sp.load(ring, win_rows, ring_rows=ring_rows, row0=0, bandpass=True, stream=s)
             tune = sp.autotune(win_rows, out, seg_rows, iters=2, stream=s)
             log(f"{sname}: autotuned tile {vfrb.tile_names()[sp.tile]}: " +
                 " ".join(f"{k}={v:.2f}" for k, v in tune.items()))
-        names = ["filterbank", "bandpass_pack", "dedisperse", "dm_stats", "candidates"]
-
-        def front(i, st):
-            fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=(max_delay + i * seg_rows) % ring_rows,
-                        stream=st)
 
         def back(i, st, marks=None):
             sp.load(ring, win_rows, ring_rows=ring_rows, row0=(i * seg_rows) % ring_rows, bandpass=True, stream=st)
@@ -186,17 +230,19 @@ def main():
                       ct.ptr, csnr.ptr, maxc, ccount.ptr, st)
 
         # (1) serial steps on one stream: per-kernel times, free of overlap
-        ev = [vfrb.Event() for _ in range(6)]
+        ne = len(names) + 1
+        ev = [vfrb.Event() for _ in range(ne)]
         stage_ms = {k: [] for k in names}
         nserial = max(2, min(args.steps, 4))
+        nf = 2 if frames_mode else 1  # front stages
         for i in range(args.warmup + nserial):
             ev[0].record(s)
-            front(i, s)
-            ev[1].record(s)
-            back(i, s, ev[2:5])
-            ev[5].record(s)
+            front(i, s, ev[1] if frames_mode else None)
+            ev[nf].record(s)
+            back(i, s, ev[nf + 1:nf + 4])
+            ev[ne - 1].record(s)
             if i >= args.warmup:
-                ev[5].sync()
+                ev[ne - 1].sync()
                 for k, nm in enumerate(names):
                     stage_ms[nm].append(ev[k].elapsed_ms(ev[k + 1]))
         stream.sync()
@@ -257,6 +303,8 @@ def main():
     in_bytes = 2.0 * seg_rows * nave * n
     esz = {"f32": 4, "f16": 2, "u8": 1}[args.storage]
     work = {
+        # frame headers and the slot table: 2 x 32 B read per pair + 4 B table write
+        "ingest": dict(bytes=npairs * (2 * 32.0 + 4.0)),
         "filterbank": dict(bytes=in_bytes + seg_rows * nch * 4.0,
                            flops=2 * 2.5 * n * np.log2(n) * seg_rows * nave),
         "bandpass_pack": dict(bytes=win_rows * nch * (4.0 * (1 if args.storage == "f32" else 2) + esz)),
@@ -276,7 +324,7 @@ def main():
                     frac=ach / FP32_PEAK_TFLOPS, kernel=kname,
                     note="no MFMA: FP32 vector-add bound; 1 add = 1 flop; adds = ndm*nch*rows_out per launch")
     else:
-        kname = {"filterbank": f"fb_pow2_kernel<{n}>"}.get(dom, dom)
+        kname = {"filterbank": f"fb_pow2_kernel<{n}, {'true' if frames_mode else 'false'}>"}.get(dom, dom)
         ach = work[dom]["bytes"] / dur_s / 1e9
         roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                     kernel=kname)
@@ -295,7 +343,7 @@ def main():
         stages_roof[k] = e
 
     # free the step's device buffers before the extra sections
-    del d0, d1, ring, out, cdm, ct, csnr
+    del d0, d1, raw, ring, out, cdm, ct, csnr
     extra = {}
     for sec in [x for x in args.sections.split(",") if x]:
         fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving}[sec]
@@ -329,7 +377,8 @@ def main():
                    "fft": n, "nave": nave, "tsamp_s": tchan, "segment_s": W["seg_s"],
                    "segment_rows": seg_rows, "max_delay_rows": max_delay, "window_rows": win_rows,
                    "storage": args.storage, "accumulate": "f32",
-                   "input": "VDIF payload resident in HBM (device-resident RTF)",
+                   "input": ("raw VDIF frame pairs resident in HBM, ingested on the GPU (device-resident RTF)"
+                             if frames_mode else "de-framed VDIF payload resident in HBM (device-resident RTF)"),
                    "schedule": "segments pipelined on two HIP streams (filterbank of i+1 beside dedispersion of i)",
                    "parallelism": "single GPU"},
         "p50_latency_ms": main_r["p50_ms"],

@@ -325,7 +325,52 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
     uint32_t x0[8], x1[8];
     auto fetch = [&](int it) {
         const int frame = it + slot;
-        if (frame < nave && FRAMED) {
+        if (frame >= nave) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) x0[r] = x1[r] = 0x80808080u;  // 128 -> 0
+            return;
+        }
+        if constexpr (FRAMED && TPF >= 64) {
+            // One FFT frame per wave: the wave resolves the few payload slots
+            // its N bytes span (scalar loads of the table), every lane picks
+            // the slot of each of its dwords by comparison.
+            constexpr int MAXS = (N + kPayloadBytes - 1) / kPayloadBytes + 1;
+            const int64_t of = byte0 + ((int64_t)row * nave + frame) * N;
+            int64_t s0 = (int64_t)((double)of * (1.0 / kPayloadBytes));
+            int w0 = (int)(of - s0 * kPayloadBytes);
+            if (w0 < 0) {
+                s0--;
+                w0 += kPayloadBytes;
+            } else if (w0 >= kPayloadBytes) {
+                s0++;
+                w0 -= kPayloadBytes;
+            }
+            s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)(s0 >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)s0);
+            w0 = __builtin_amdgcn_readfirstlane(w0);
+            const int nspan = (w0 + N - 1) / kPayloadBytes;  // last slot index used, < MAXS
+            int ks[MAXS];
+#pragma unroll
+            for (int j = 0; j < MAXS; j++) ks[j] = j <= nspan ? table[s0 + j] : -1;
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int pos = w0 + 4 * t + r * EIGHTH;
+                int j = 0;
+#pragma unroll
+                for (int m = 1; m < MAXS; m++) j += pos >= m * kPayloadBytes ? 1 : 0;
+                int k = ks[0];
+#pragma unroll
+                for (int m = 1; m < MAXS; m++) k = j == m ? ks[m] : k;
+                uint32_t a0 = 0u, a1 = 0u;  // zero-filled payload of a missing frame
+                if (k >= 0) {
+                    const uint8_t* f = pol0 + (int64_t)k * (2 * kFrameBytes) + kHeaderBytes + (pos - j * kPayloadBytes);
+                    a0 = *reinterpret_cast<const uint32_t*>(f);
+                    a1 = *reinterpret_cast<const uint32_t*>(f + kFrameBytes);
+                }
+                x0[r] = a0;
+                x1[r] = a1;
+            }
+        } else if constexpr (FRAMED) {
             // payload offset -> (slot, byte): one FP64 division, then steps of
             // N/8 < 5000 bytes carry into the next slot at most once
             const int64_t o = byte0 + ((int64_t)row * nave + frame) * N + 4 * t;
@@ -341,20 +386,21 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 const int k = table[sl];
+                uint32_t a0 = 0u, a1 = 0u;  // zero-filled payload of a missing frame
                 if (k >= 0) {
                     const uint8_t* f = pol0 + (int64_t)k * (2 * kFrameBytes) + kHeaderBytes + w;
-                    x0[r] = *reinterpret_cast<const uint32_t*>(f);
-                    x1[r] = *reinterpret_cast<const uint32_t*>(f + kFrameBytes);
-                } else {
-                    x0[r] = x1[r] = 0u;  // zero-filled payload of a missing frame
+                    a0 = *reinterpret_cast<const uint32_t*>(f);
+                    a1 = *reinterpret_cast<const uint32_t*>(f + kFrameBytes);
                 }
+                x0[r] = a0;
+                x1[r] = a1;
                 w += EIGHTH;
                 if (w >= kPayloadBytes) {
                     w -= kPayloadBytes;
                     sl++;
                 }
             }
-        } else if (frame < nave) {
+        } else {
             const uint8_t* b0 = pol0 + ((size_t)row * nave + frame) * N + 4 * t;
             const uint8_t* b1 = pol1 + ((size_t)row * nave + frame) * N + 4 * t;
 #pragma unroll
@@ -362,9 +408,6 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                 x0[r] = *reinterpret_cast<const uint32_t*>(b0 + r * EIGHTH);
                 x1[r] = *reinterpret_cast<const uint32_t*>(b1 + r * EIGHTH);
             }
-        } else {
-#pragma unroll
-            for (int r = 0; r < 8; r++) x0[r] = x1[r] = 0x80808080u;  // 128 -> 0
         }
     };
     fetch(0);
