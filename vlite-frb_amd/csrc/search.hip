@@ -197,6 +197,7 @@ __global__ void range_init_kernel(unsigned* range, int nch) {
 // Dedispersion
 // ---------------------------------------------------------------------------
 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 constexpr h2_t kSelLo = {(_Float16)1.0f, (_Float16)0.0f};
 constexpr h2_t kSelHi = {(_Float16)0.0f, (_Float16)1.0f};
 
@@ -210,11 +211,34 @@ struct Chunk {
 // and two fit in one CU's 160 KB of LDS.
 constexpr int kChunkMax = 32;
 constexpr size_t kDataBudget = 60 * 1024;
-constexpr int kStageG = 2;  // 4-dword staging groups in flight per lane
+constexpr int kBufferFlags = 0x00020000;  // word 3 of a gfx9 raw buffer resource
 
-// LDS (<= 76 KB per workgroup) limits a CU to two workgroups, i.e. W/2 waves
-// per SIMD: let the register allocator use the VGPR budget that occupancy
-// implies (512 / (W/2) per lane), so that many LDS reads can be in flight.
+// LDS ahead of the staged rows: the chunk's [kChunkMax][DMB] u32 read offsets,
+// then two [kChunkMax] buffers of channel start rows (dmin), one per chunk
+// parity, so that chunk k+1's are loaded while chunk k is staged.
+__host__ __device__ constexpr size_t lds_head_bytes(int dmb) {
+    return (size_t)kChunkMax * dmb * 4 + 2 * kChunkMax * 4;
+}
+
+// Staging items (channel, 4-dword group) each thread loads per pass: enough
+// for the largest chunk the data budget admits (kDataBudget / (16*EPD)
+// groups) in one round trip to memory, as far as the registers allow: an
+// item holds NSRC data dwords and two offsets, and staging may take a
+// quarter of the per-lane VGPR budget of the tile's occupancy (1024/W).
+template <int EPD, int W>
+struct StageItems {
+    static constexpr int groups = (int)(kDataBudget / (16 * EPD));
+    static constexpr int need = (groups + 64 * W - 1) / (64 * W);
+    static constexpr int fit = (1024 / W / 4) / ((EPD > 1 ? 5 : 4) + 2);
+    static constexpr int cap = fit < 1 ? 1 : (fit > 8 ? 8 : fit);
+    static constexpr int value = need < cap ? need : cap;
+};
+
+// LDS (<= 77 KB per workgroup) limits a CU to two workgroups, i.e. W/2 waves
+// per SIMD.  The kernels ask for exactly that occupancy (both bounds of
+// amdgpu_waves_per_eu): the register allocator then fills the VGPR budget it
+// implies (512 / (W/2) per lane), so that many LDS reads are in flight, and
+// never exceeds it, which would leave room for only one workgroup per CU.
 template <int W>
 struct WavesPerEu {
     static constexpr int value = W / 2 > 1 ? W / 2 : 1;
@@ -242,13 +266,21 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     constexpr int DMB = W * DW;
     constexpr int NDW = SL / EPD;          // dwords per lane per (DM, channel)
     static_assert(SL % EPD == 0, "SL must hold whole dwords");
+    constexpr int NT = 64 * W;              // threads per workgroup
+    constexpr int NSRC = EPD > 1 ? 5 : 4;   // source dwords per staged group
+    constexpr int NI = StageItems<EPD, W>::value;
+    constexpr int NR = (kChunkMax * DMB / 4 + NT - 1) / NT;  // offset-table uint4 per thread
     uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
-    unsigned char* data = lds_raw + kChunkMax * DMB * 4;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int* dbuf = reinterpret_cast<int*>(lds_raw + (size_t)kChunkMax * DMB * 4);
+    unsigned char* data = lds_raw + lds_head_bytes(DMB);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int t0 = tx * T;
     const int dm0 = b * DMB + wave * DW;
     const int2 cr = blk_chunks[b];
+    const int* dmin_b = dmin + (int64_t)b * nch;
+    const uint32_t ldw = (uint32_t)(ld / EPD);  // dwords per window row
 
     float acc[DW][SL];
 #pragma unroll
@@ -256,58 +288,105 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
 #pragma unroll
         for (int j = 0; j < SL; j++) acc[d][j] = 0.0f;
 
+    // first chunk's channel start rows
+    if (cr.y > 0) {
+        const Chunk c0 = chunks[cr.x];
+        if (tid < c0.count) dbuf[tid] = dmin_b[c0.c0 + tid];
+    }
+    __syncthreads();
+
     for (int k = 0; k < cr.y; k++) {
         const Chunk ch = chunks[cr.x + k];
         const int rowb = 4 * ch.stride;    // bytes per staged channel (all copies)
-        const int nsd = ch.stride / EPD;   // dwords per copy, a multiple of 4
-        // Stage the channel's row from its dword-aligned start a0 = r0 - phase
-        // (the phase, dmin mod EPD, is folded into the offset table): lane l
-        // loads source dwords 4g..4g+4 of group g and writes dwords 4g..4g+3
-        // of every copy, copy p being the source shifted by p elements
-        // (v_alignbyte across neighbouring dwords).  Dwords past the row end
-        // are clamped to it: they only feed samples t >= nt, never stored.
-        for (int ci = wave; ci < ch.count; ci += W) {
-            const int c = ch.c0 + ci;
-            const int r0 = t0 + dmin[(int64_t)b * nch + c];
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(win + (int64_t)c * ld);
-            const int sd0 = r0 / EPD;
-            const int last = (int)(ld / EPD) - 1;
-            unsigned char* row = data + ci * rowb;
-            for (int g0 = 0; g0 < nsd / 4; g0 += 64 * kStageG) {
-                uint32_t sv[kStageG][5];
+        const int ng = ch.stride / EPD / 4;  // 4-dword groups per copy
+        const int items = ch.count * ng;
+        const int* dcur = dbuf + (k & 1) * kChunkMax;
+        // Issue every global load of the chunk before the first LDS write, so
+        // that staging costs one round trip: the chunk's offset table, the
+        // next chunk's start rows and the staged groups.
+        // rv: byte offset of the first dword of each (channel, DM) read, built
+        // on the host: a contiguous copy of count*DMB words (DMB % 4 == 0)
+        u32x4 rv[NR];
 #pragma unroll
-                for (int u = 0; u < kStageG; u++) {
-                    const int j = sd0 + 4 * (g0 + lane + 64 * u);
+        for (int r = 0; r < NR; r++) {
+            const int i = tid + NT * r;
+            if (i < ch.count * DMB / 4) rv[r] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
+        }
+        int dnext = 0;
+        const bool more = k + 1 < cr.y;
+        Chunk nx = ch;
+        if (more) {
+            nx = chunks[cr.x + k + 1];
+            if (tid < nx.count) dnext = dmin_b[nx.c0 + tid];
+        }
+        // Item j = (channel ci, group g), ci = j / ng: a float quotient, off by
+        // at most one for j < 2^15, corrected.  Group g of channel c covers
+        // source dwords sd0 + 4g .. sd0 + 4g + 3 (+ one more for the shifted
+        // copies), sd0 the dword-aligned start of the rows the tile reads:
+        // (t0 + dmin) / EPD; the phase, dmin mod EPD, is folded into the
+        // offset table.  It is written to dwords 4g..4g+3 of every copy, copy
+        // p being the source shifted by p elements (v_alignbyte across
+        // neighbouring dwords).  The loads are buffer loads from the chunk's
+        // first window row (32-bit offsets, one VGPR per item) bounded by the
+        // window's end: dwords past a row read the next row, or zero past the
+        // last, and only feed samples t >= nt, never stored.
+        const uint32_t* base = reinterpret_cast<const uint32_t*>(win) + (int64_t)ch.c0 * ldw;
+        const int64_t left = ((int64_t)nch - ch.c0) * ldw * 4;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<uint32_t*>(base), 0, (int)(left < 0x7fffffff ? left : 0x7fffffff), kBufferFlags);
+        const float rng = 1.0f / (float)ng;
+        auto stage = [&](int j0) {
+            uint32_t vo[NI], sv[NI][NSRC];
+            int wo[NI];
 #pragma unroll
-                    for (int e = 0; e < (EPD > 1 ? 5 : 4); e++) sv[u][e] = src[min(j + e, last)];
-                }
+            for (int m = 0; m < NI; m++) {
+                const int j = j0 + tid + NT * m;
+                int ci = (int)((float)j * rng);
+                int g = j - ci * ng;
+                const int fix = (g >> 31) + (g >= ng ? 1 : 0);  // -1, 0 or +1, branch-free
+                ci += fix;
+                g -= fix * ng;
+                const bool ok = j < items;
+                const uint32_t r0 = (uint32_t)(t0 + dcur[ok ? ci : 0]);
+                // an idle item reads at offset 2^31, past the resource: zeros
+                vo[m] = ok ? 4u * ((uint32_t)ci * ldw + r0 / EPD + 4u * (uint32_t)g) : 0x80000000u;
+                wo[m] = ok ? ci * rowb + 16 * g : -1;
+            }
 #pragma unroll
-                for (int u = 0; u < kStageG; u++) {
-                    const int g = g0 + lane + 64 * u;
-                    if (4 * g >= nsd) continue;
+            for (int m = 0; m < NI; m++)
 #pragma unroll
-                    for (int p = 0; p < EPD; p++) {
-                        uint4 w;
-                        if (p == 0) {
-                            w = make_uint4(sv[u][0], sv[u][1], sv[u][2], sv[u][3]);
-                        } else {
-                            w.x = __builtin_amdgcn_alignbyte(sv[u][1], sv[u][0], p * ESZ);
-                            w.y = __builtin_amdgcn_alignbyte(sv[u][2], sv[u][1], p * ESZ);
-                            w.z = __builtin_amdgcn_alignbyte(sv[u][3], sv[u][2], p * ESZ);
-                            w.w = __builtin_amdgcn_alignbyte(sv[u][4], sv[u][3], p * ESZ);
-                        }
-                        *reinterpret_cast<uint4*>(row + p * ch.stride * ESZ + 16 * g) = w;
+                for (int e = 0; e < NSRC; e++)
+                    sv[m][e] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[m] + 4 * e, 0, 0);
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                if (wo[m] < 0) continue;
+#pragma unroll
+                for (int p = 0; p < EPD; p++) {
+                    uint4 w;
+                    if (p == 0) {
+                        w = make_uint4(sv[m][0], sv[m][1], sv[m][2], sv[m][3]);
+                    } else {
+                        w.x = __builtin_amdgcn_alignbyte(sv[m][1], sv[m][0], p * ESZ);
+                        w.y = __builtin_amdgcn_alignbyte(sv[m][2], sv[m][1], p * ESZ);
+                        w.z = __builtin_amdgcn_alignbyte(sv[m][3], sv[m][2], p * ESZ);
+                        w.w = __builtin_amdgcn_alignbyte(sv[m][4], sv[m][3], p * ESZ);
                     }
+                    *reinterpret_cast<uint4*>(data + wo[m] + p * ch.stride * ESZ) = w;
                 }
             }
+        };
+        // the first pass shares its round trip with the offsets; one pass
+        // covers any chunk when the item count allows it
+        stage(0);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int i = tid + NT * r;
+            if (i < ch.count * DMB / 4) reinterpret_cast<u32x4*>(roff)[i] = rv[r];
         }
-        // byte offset of the first dword of each (channel, DM) read, built on
-        // the host: a contiguous copy of count*DMB words (DMB % 4 == 0)
-        {
-            const uint4* src = reinterpret_cast<const uint4*>(roffs + ch.roff);
-            uint4* dst = reinterpret_cast<uint4*>(roff);
-            for (int i = threadIdx.x; i < ch.count * DMB / 4; i += 64 * W) dst[i] = src[i];
+        if constexpr (StageItems<EPD, W>::need > StageItems<EPD, W>::value) {
+            for (int j0 = NT * NI; j0 < items; j0 += NT * NI) stage(j0);
         }
+        if (more && tid < nx.count) dbuf[((k + 1) & 1) * kChunkMax + tid] = dnext;
         __syncthreads();
         for (int ci = 0; ci < ch.count; ci++) {
             const uint32_t* ro = roff + ci * DMB + wave * DW;
@@ -379,7 +458,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
 
 // One workgroup per tile: 1-D grid of nblk * (time tiles) workgroups.
 template <typename TS, int SL, int DW, int W>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
               const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
               const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
@@ -420,7 +499,7 @@ struct BatchTile {
 // puts both back to zero, so the next launch (or graph replay) starts clean
 // without a memset; ntiles[0] = tiles in the list.
 template <typename TS, int SL, int DW, int W>
-__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(1, WavesPerEu<W>::value)))
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restrict__ tiles,
                     const int* __restrict__ ntiles, unsigned* __restrict__ ctr) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -676,7 +755,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     p->nblk = nblk;
     p->tile = i;
     p->blk_cost = std::move(cost);
-    p->lds_bytes = (size_t)kChunkMax * DMB * 4 + max_data;
+    p->lds_bytes = lds_head_bytes(DMB) + max_data;
     return VFRB_OK;
 }
 
@@ -742,6 +821,11 @@ vfrb_status vfrb_search_plan_create(vfrb_search_plan** plan, const vfrb_search_d
     }
     // Window rows padded to a multiple of 64 elements (256-byte rows in f32).
     p->ld = ((int64_t)d->max_rows + 63) / 64 * 64;
+    if ((int64_t)kChunkMax * p->ld * (int64_t)storage_bytes(p->storage) >= (1LL << 31)) {
+        delete p;
+        vfrb::set_error("max_rows %d too long: %d window rows must stay under 2 GiB", d->max_rows, kChunkMax);
+        return VFRB_ERR_UNSUPPORTED;
+    }
     const size_t win_bytes = (size_t)p->nch * p->ld * storage_bytes(p->storage);
     const int nbpblk = (d->max_rows + BP_ROWS - 1) / BP_ROWS;
     hipError_t e = hipMalloc(&p->d_off, p->off.size() * sizeof(int32_t));
@@ -942,7 +1026,7 @@ void drop_graph(vfrb_batch* b) {
 // captured launch serves every job set.
 vfrb_status batch_configure(vfrb_batch* b, int i) {
     const TileCfg& tc = kTiles[i];
-    const size_t lds = (size_t)kChunkMax * tc.w * tc.dw * 4 + kDataBudget;
+    const size_t lds = lds_head_bytes(tc.w * tc.dw) + kDataBudget;
     const void* kern = tile_kernel_for(b->storage, i, true);
     VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int dev = 0, per_cu = 0, cus = 0;
