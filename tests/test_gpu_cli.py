@@ -38,7 +38,8 @@ def test_pipeline_cli_matches_oracle(gpu, orc, tmp_path):
     print(f"[case] CLI maxnt={p.maxnt} max_delay={maxd} ntime={ntime}", flush=True)
     r = subprocess.run([PIPE, "-d", "-nchan", str(nchan), "-nave", str(nave), "-nb", str(nb), "-i", "in.vdif",
                         "-o", "run", "--fullband", "--window", str(window), "--dump-window", "1",
-                        "--cands", "run.cands"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+                        "--cands", "run.cands", "--fil-stream", "run.fil"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
     assert r.returncode == 0, r.stderr + open(tmp_path / "run.log").read()
     assert "TOTAL TIME" in r.stdout
     data = open(tmp_path / "in.vdif", "rb").read()
@@ -56,6 +57,15 @@ def test_pipeline_cli_matches_oracle(gpu, orc, tmp_path):
     assert struct.pack("<d", p.fch1) in head and struct.pack("<d", p.tchan) in head
     cands = open(tmp_path / "run.cands").read().splitlines()
     assert cands[0].startswith("#")
+    # streamed .fil: the same header, then every window's bandpassed rows from
+    # where its filling started (row 0, then (max_delay/NBUNDLE)*NBUNDLE)
+    stream = open(tmp_path / "run.fil", "rb").read()
+    assert stream.startswith(head)
+    rows = np.frombuffer(stream[len(head):], np.float32).reshape(-1, p.nchanout)
+    r0 = (maxd // nb) * nb
+    ref = np.concatenate([orc.bandpass_f64(w)[0 if i == 0 else r0:] for i, w in enumerate(wins)])
+    assert rows.shape == ref.shape
+    assert np.max(np.abs(rows - ref)) / ref.std() < 1e-3
 
 
 def test_timeseries_cli_matches_oracle(gpu, orc, tmp_path):

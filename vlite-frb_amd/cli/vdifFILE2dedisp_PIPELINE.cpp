@@ -12,6 +12,9 @@
 //   --storage T       f32 (default) | f16 | u8 filterbank storage
 //   --dump-window Q   window whose filterbank/series are written (default 4)
 //   --cands FILE      write every (window, DM, sample, SNR) above 7 sigma
+//   --fil-stream FILE SIGPROC .fil (header + data) of every window: the
+//                     bandpassed rows each window adds (all rows of the
+//                     first), so each input sample appears once, in order
 //   --device N
 #include "cli_common.hpp"
 
@@ -26,6 +29,7 @@ struct Args {
     int storage = VFRB_STORE_F32;
     int dump_window = 4;
     std::string cands;
+    std::string fil_stream;
     int device = 0;
 };
 
@@ -40,7 +44,8 @@ void print_usage() {
     std::fprintf(stderr, "\t             -nchan (int):  number of frequency channels,\n");
     std::fprintf(stderr, "\t             -nb (int):     number of bundles of size (2*2*nchan*nave) to send in 1 memcpy\n");
     std::fprintf(stderr, "\t             --fullband, --window SEC, --dmlist FILE, --storage f32|f16|u8,\n");
-    std::fprintf(stderr, "\t             --dump-window Q, --cands FILE, --device N   (MI355X build extensions)\n");
+    std::fprintf(stderr, "\t             --dump-window Q, --cands FILE, --fil-stream FILE, --device N\n");
+    std::fprintf(stderr, "\t             (MI355X build extensions)\n");
     std::fprintf(stderr, "\n");
     std::fprintf(stderr, "  DEFAULTS:\n");
     std::fprintf(stderr, "\t             -i     %s\n", "/data/VLITE/0329+54/20150302_151845_B0329+54_ev_0005.out.uw");
@@ -77,6 +82,7 @@ int parse(int argc, char** argv, Args& a) {
         else if (cli::eq(s, "--window")) a.window = std::atof(next());
         else if (cli::eq(s, "--dmlist")) a.dmlist = next();
         else if (cli::eq(s, "--dump-window")) a.dump_window = std::atoi(next());
+        else if (cli::eq(s, "--fil-stream")) a.fil_stream = next();
         else if (cli::eq(s, "--cands")) a.cands = next();
         else if (cli::eq(s, "--device")) a.device = std::atoi(next());
         else if (cli::eq(s, "--storage")) {
@@ -254,6 +260,17 @@ int main(int argc, char** argv) {
     int32_t hlen = 0;
     CLI_CHECK(vfrb_sigproc_header(hbuf, sizeof hbuf, &hlen, p.fch1, p.foff, p.nchanout, 32, tstart, tsamp), 2);
     std::fwrite(hbuf, 1, hlen, fhead);
+    FILE* fstream = nullptr;
+    std::vector<float> stream_fb;
+    if (!a.fil_stream.empty()) {
+        fstream = std::fopen(a.fil_stream.c_str(), "wb");
+        if (!fstream) {
+            std::fprintf(stderr, "Unable to open SIGPROC filterbank stream: %s\n", a.fil_stream.c_str());
+            return 2;
+        }
+        std::fwrite(hbuf, 1, hlen, fstream);
+        stream_fb.resize((size_t)p.maxnt * p.nchanout);
+    }
 
     // Main loop (PIPELINE:559-708): one bundle per iteration, a window when
     // MAXNT/NBUNDLE bundles are in.
@@ -274,9 +291,15 @@ int main(int argc, char** argv) {
                     loopcnt * p.nbundle);
             fprintf(flog, "executing dedisp plan...\n");
             CLI_CHECK(vfrb_search_load(sp, dfb, 0, 0, p.maxnt, 1, nullptr), 1);
-            if (q == a.dump_window) {
-                CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
-                CLI_CHECK(vfrb_memcpy_d2h(host_fb.data(), dfb, fb_bytes, nullptr), 2);
+            if (q == a.dump_window || fstream) CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
+            if (q == a.dump_window) CLI_CHECK(vfrb_memcpy_d2h(host_fb.data(), dfb, fb_bytes, nullptr), 2);
+            if (fstream) {
+                // rows this window adds: filling restarts at (max_delay/NBUNDLE)*NBUNDLE
+                const int r0 = q == 0 ? 0 : (max_delay / p.nbundle) * p.nbundle;
+                const size_t n = (size_t)(p.maxnt - r0) * p.nchanout;
+                CLI_CHECK(vfrb_memcpy_d2h(stream_fb.data(), dfb + (size_t)r0 * p.nchanout, n * sizeof(float),
+                                          nullptr), 2);
+                std::fwrite(stream_fb.data(), sizeof(float), n, fstream);
             }
             CLI_CHECK(vfrb_search_dedisperse(sp, p.maxnt, ddd, ntime, nullptr), 1);
             CLI_CHECK(vfrb_dm_stats(ddd, ntime, ndm, ntime, acc, dfrb, dave, dsig, dpeak, dpt, nullptr), 1);
@@ -322,6 +345,7 @@ int main(int argc, char** argv) {
     std::fclose(ffil);
     std::fclose(fdd);
     if (fcand) std::fclose(fcand);
+    if (fstream) std::fclose(fstream);
     vfrb_seq_close(seq);
     vfrb_fb_plan_destroy(fbp);
     vfrb_search_plan_destroy(sp);
