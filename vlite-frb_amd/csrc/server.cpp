@@ -131,6 +131,7 @@ struct Lane {
     hipStream_t s = nullptr;
     vfrb_batch* batch = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_copy = nullptr;  // this lane's host-to-device copies done (copy stream)
     bool busy = false;
     bool issued = false;  // ev1 recorded for the current iteration
     double start = 0.0;
@@ -148,6 +149,11 @@ struct vfrb_server {
     Arena arena;
     std::vector<std::unique_ptr<SrvStream>> streams;
     std::vector<Lane> lanes;
+    // With two lanes, every host-to-device copy goes through this one stream
+    // in issue order: concurrent copies from both lanes would share the PCIe
+    // link and delay each other, while one queue keeps it busy back to back
+    // and the lanes' kernels overlap the next copies.
+    hipStream_t copy_s = nullptr;
     std::deque<Request> pending;
     std::deque<vfrb_server_result> completed;
     std::mutex mu;
@@ -193,7 +199,29 @@ void issue(vfrb_server* S, Lane& L) {
     hipStream_t s = L.s;
     (void)hipEventRecord(L.ev0, s);
     const int64_t rb = S->row_bytes;
+    // host input: the copies first, on the shared copy stream; the lane waits
+    bool host_copied = false;
+    if (S->copy_s) {
+        for (Item& it : L.items) {
+            if (it.req.device) continue;
+            const int64_t C = it.st->carry_bytes;
+            for (int p = 0; p < 2 && it.status == VFRB_OK; p++)
+                it.status = vfrb_memcpy_h2d(S->arena_base + it.off[p] + C, it.req.pol[p], (size_t)it.req.nbytes,
+                                            S->copy_s);
+            host_copied = true;
+        }
+        if (host_copied) {
+            hipError_t he = hipEventRecord(L.ev_copy, S->copy_s);
+            if (he == hipSuccess) he = hipStreamWaitEvent(s, L.ev_copy, 0);
+            if (he != hipSuccess) {
+                const vfrb_status e = vfrb::hip_fail(he, "server copy queue", __FILE__, __LINE__);
+                for (Item& it : L.items)
+                    if (it.status == VFRB_OK) it.status = e;
+            }
+        }
+    }
     for (Item& it : L.items) {
+        if (it.status != VFRB_OK) continue;
         SrvStream* st = it.st;
         const int64_t C = st->carry_bytes, tot = C + it.req.nbytes;
         const int64_t R = tot / rb, newc = tot - R * rb;
@@ -201,9 +229,10 @@ void issue(vfrb_server* S, Lane& L) {
         vfrb_status e = VFRB_OK;
         for (int p = 0; p < 2 && e == VFRB_OK; p++) {
             if (C > 0) e = vfrb_memcpy_d2d(reg[p], st->carry + p * rb, (size_t)C, s);
-            if (e == VFRB_OK)
-                e = it.req.device ? vfrb_memcpy_d2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s)
-                                  : vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+            if (e == VFRB_OK && it.req.device)
+                e = vfrb_memcpy_d2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+            else if (e == VFRB_OK && !host_copied)
+                e = vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
         }
         if (e == VFRB_OK && R > 0)
             e = vfrb_fb_execute(S->fbp, reg[0], reg[1], R, st->ring, st->ring_rows, st->total_rows, s);
@@ -436,8 +465,10 @@ void server_free(vfrb_server* S) {
         if (l.batch) vfrb_batch_destroy(l.batch);
         if (l.ev0) (void)hipEventDestroy(l.ev0);
         if (l.ev1) (void)hipEventDestroy(l.ev1);
+        if (l.ev_copy) (void)hipEventDestroy(l.ev_copy);
         if (l.s) (void)hipStreamDestroy(l.s);
     }
+    if (S->copy_s) (void)hipStreamDestroy(S->copy_s);
     if (S->fbp) vfrb_fb_plan_destroy(S->fbp);
     (void)hipFree(S->arena_base);
     delete S;
@@ -467,12 +498,14 @@ vfrb_status vfrb_server_create(vfrb_server** server, const vfrb_server_desc* d) 
     if (e == hipSuccess && st == VFRB_OK) e = hipMalloc(&S->arena_base, (size_t)d->arena_bytes);
     S->arena.reset(d->arena_bytes);
     const int nlanes = d->batched ? 2 : 1;
+    if (e == hipSuccess && st == VFRB_OK && nlanes > 1) e = hipStreamCreateWithFlags(&S->copy_s, hipStreamNonBlocking);
     S->lanes.resize(nlanes);
     for (Lane& l : S->lanes) {
         if (e != hipSuccess || st != VFRB_OK) break;
         e = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&l.ev0);
         if (e == hipSuccess) e = hipEventCreate(&l.ev1);
+        if (e == hipSuccess && nlanes > 1) e = hipEventCreateWithFlags(&l.ev_copy, hipEventDisableTiming);
         if (e == hipSuccess && d->batched)
             st = vfrb_batch_create(&l.batch, d->storage, d->tile_config, d->max_streams, 1 << 22);
     }
