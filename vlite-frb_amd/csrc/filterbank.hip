@@ -327,7 +327,10 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         const int frame = it + slot;
         if (frame >= nave) {
 #pragma unroll
-            for (int r = 0; r < 8; r++) x0[r] = x1[r] = 0x80808080u;  // 128 -> 0
+            for (int r = 0; r < 8; r++) {  // 128 -> 0
+                x0[r] = 0x80808080u;
+                x1[r] = 0x80808080u;
+            }
             return;
         }
         if constexpr (FRAMED && TPF >= 64) {
