@@ -125,7 +125,7 @@ def test_delay_table_device_matches_oracle(gpu, orc):
     assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
 
 
-@pytest.mark.parametrize("tile", range(12))
+@pytest.mark.parametrize("tile", range(16))
 def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()
@@ -142,6 +142,36 @@ def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     ref = orc.dedisperse_f32(fb, plan.offsets(), nt)
     # same terms, different summation tree: f32 rounding only
     assert np.max(np.abs(got - ref)) < 1e-4 * 22
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_pair_tiles_match_channel_tiles(gpu, storage):
+    """The pair-presum tiles add the same terms as the per-channel tiles, each
+    channel pair first: equal to f32 rounding on the same stored window.  The
+    u8 accumulator holds sum(scale*byte), of the size of the channel offsets
+    subtracted afterwards (about 60x the output here), so its summation-order
+    rounding is that much larger relative to the output."""
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(31)
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[storage], 3)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.load(d, rows, bandpass=True)
+    plan.dedisperse(rows, out, nt)
+    ref = out.download()
+    names = vfrb.tile_names()
+    for i in (i for i, n in enumerate(names) if n.startswith("P")):
+        print(f"[case] pair tile {names[i]} {storage}", flush=True)
+        plan.set_tile(i)
+        out.zero()
+        plan.dedisperse(rows, out, nt)
+        got = out.download()
+        tol = (1e-3 if storage == "u8" else 1e-5) * np.max(np.abs(ref))
+        assert np.max(np.abs(got - ref)) <= tol, (names[i], storage, np.max(np.abs(got - ref)))
 
 
 def test_autotune_picks_a_fitting_tile_and_keeps_results(gpu, orc):

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include <hip/hip_fp16.h>
@@ -204,6 +205,8 @@ constexpr h2_t kSelHi = {(_Float16)0.0f, (_Float16)1.0f};
 struct Chunk {
     int c0, count, stride;  // stride: staged samples per copy, multiple of 4
     int roff;               // first entry of this chunk in the LDS byte-offset table
+    int nd;                 // pair tiles: presum rows per channel pair (else 1)
+    int pad[3];
 };
 
 // At most kChunkMax channels per LDS chunk; staged rows take at most
@@ -443,6 +446,234 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pair tiles.  Channels c and c+1 of a chunk are summed once into f32 presum
+// rows, P_i[s] = x[c][S + s] + x[c+1][S + s + dlo + i], one row per relative
+// delay dlo + i that the block's DMs use for the pair (off(d,c+1) - off(d,c)
+// varies by a sample or two across 64 adjacent DMs).  Every DM then adds one
+// presum per pair, out[d][t] += P_{delta(d)-dlo}[t - t0 + off(d,c) - S + t0]:
+// half the adds of the per-channel tile, and an f32 read per sample whatever
+// the window storage.  The terms are the same; only the summation tree
+// differs (each pair is added first).
+// ---------------------------------------------------------------------------
+constexpr int kPairMax = kChunkMax / 2;  // channel pairs per chunk
+constexpr int kPairMaxND = 4;            // presum rows per pair at most
+
+// Staging items (pair, 4-sample group) per thread per pass: each holds up to
+// 2 + (4 + kPairMaxND - 1)/EPD + 2 source dwords and two offsets; staging may
+// take a quarter of the tile's per-lane VGPR budget (1024/W).
+template <int EPD, int W>
+struct PairStageItems {
+    static constexpr int groups = (int)(kDataBudget / 16);  // 16 B per item and presum row
+    static constexpr int need = (groups + 64 * W - 1) / (64 * W);
+    static constexpr int regs = 4 + (4 + kPairMaxND - 1) + 2;  // f32 elements of both channels, offsets
+    static constexpr int fit = (1024 / W / 4) / regs;
+    static constexpr int cap = fit < 1 ? 1 : (fit > 8 ? 8 : fit);
+    static constexpr int value = need < cap ? need : cap;
+};
+
+// K consecutive window elements of one channel as f32 (u8: times the channel
+// scale), from element e of the row whose first dword is at byte voff of the
+// resource.  A missing channel (voff >= 2^31) reads as zeros.
+template <typename TS, int K>
+__device__ __forceinline__ void load_elems(__amdgpu_buffer_rsrc_t rs, uint32_t voff, int phase, float sc,
+                                           float (&v)[K]) {
+    constexpr int ESZ = (int)sizeof(TS);
+    constexpr int EPD = 4 / ESZ;
+    constexpr int NW = (K + EPD - 1) / EPD;  // shifted dwords needed
+    uint32_t dw[NW + 1];
+#pragma unroll
+    for (int i = 0; i < NW + 1; i++) dw[i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, voff + 4 * i, 0, 0);
+    if constexpr (ESZ == 4) {
+#pragma unroll
+        for (int i = 0; i < K; i++) v[i] = __uint_as_float(dw[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NW; i++) {
+            const uint32_t w = __builtin_amdgcn_alignbyte(dw[i + 1], dw[i], phase * ESZ);
+#pragma unroll
+            for (int e = 0; e < EPD; e++) {
+                const int k = EPD * i + e;
+                if (k >= K) break;
+                if constexpr (ESZ == 2) {
+                    const h2_t h = *reinterpret_cast<const h2_t*>(&w);
+                    v[k] = (float)h[e];
+                } else {
+                    v[k] = sc * (float)((w >> (8 * e)) & 0xffu);
+                }
+            }
+        }
+    }
+}
+
+template <typename TS, int SL, int DW, int W>
+__device__ __forceinline__ void
+dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+                 const uint32_t* __restrict__ roffs,  // per chunk: [pairs][W*DW] LDS byte offsets
+                 const int* __restrict__ dmin,        // [nblk][nch]: (start row, dlo) per pair
+                 const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+                 const float* __restrict__ chan_scale, const int* __restrict__ f16_exp,
+                 const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out, int b, int tx,
+                 unsigned char* lds_raw) {
+    constexpr int ESZ = (int)sizeof(TS);
+    constexpr int EPD = 4 / ESZ;
+    constexpr int T = 64 * SL;
+    constexpr int DMB = W * DW;
+    constexpr int NT = 64 * W;
+    constexpr int K2 = 4 + kPairMaxND - 1;  // elements of the second channel per group
+    constexpr int NI = PairStageItems<EPD, W>::value;
+    constexpr int NR = (kPairMax * DMB / 4 + NT - 1) / NT;
+    uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
+    int* dbuf = reinterpret_cast<int*>(lds_raw + (size_t)kChunkMax * DMB * 4);
+    unsigned char* data = lds_raw + lds_head_bytes(DMB);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t0 = tx * T;
+    const int dm0 = b * DMB + wave * DW;
+    const int2 cr = blk_chunks[b];
+    const int* dmin_b = dmin + (int64_t)b * nch;
+    const uint32_t ldw = (uint32_t)(ld / EPD);
+
+    float acc[DW][SL];
+#pragma unroll
+    for (int d = 0; d < DW; d++)
+#pragma unroll
+        for (int j = 0; j < SL; j++) acc[d][j] = 0.0f;
+
+    // dbuf[parity][2p] = start row of pair p's first channel, [2p+1] = dlo
+    if (cr.y > 0) {
+        const Chunk c0 = chunks[cr.x];
+        if (tid < 2 * c0.count && c0.c0 + tid < nch) dbuf[tid] = dmin_b[c0.c0 + tid];
+    }
+    __syncthreads();
+
+    for (int k = 0; k < cr.y; k++) {
+        const Chunk ch = chunks[cr.x + k];
+        const int np = ch.count, nd = ch.nd;
+        const int rowb = nd * ch.stride * 4;  // bytes per pair (nd presum rows)
+        const int ng = ch.stride / 4;         // 4-sample groups per row
+        const int items = np * ng;
+        const int* dcur = dbuf + (k & 1) * kChunkMax;
+        u32x4 rv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int i = tid + NT * r;
+            if (i < np * DMB / 4) rv[r] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
+        }
+        int dnext = 0;
+        const bool more = k + 1 < cr.y;
+        Chunk nx = ch;
+        if (more) {
+            nx = chunks[cr.x + k + 1];
+            if (tid < 2 * nx.count && nx.c0 + tid < nch) dnext = dmin_b[nx.c0 + tid];
+        }
+        const TS* base = win + (int64_t)ch.c0 * ld;
+        const int64_t left = ((int64_t)nch - ch.c0) * ld * ESZ;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<TS*>(base), 0, (int)(left < 0x7fffffff ? left : 0x7fffffff), kBufferFlags);
+        const float rng = 1.0f / (float)ng;
+        // Item j = (pair p, group g): presum samples 4g..4g+3 of each of the
+        // pair's nd rows.  The first channel's elements e1..e1+3 and the
+        // second's e1+dlo .. e1+dlo+nd+2 are loaded once (buffer loads,
+        // zeros past the window or for a missing second channel) and summed
+        // for every row.  Elements past a row only feed samples t >= nt.
+        auto stage = [&](int j0) {
+            uint32_t va[NI], vb[NI];
+            int pha[NI], phb[NI], wo[NI];
+            float sa[NI], sb[NI];
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                const int j = j0 + tid + NT * m;
+                int p = (int)((float)j * rng);
+                int g = j - p * ng;
+                const int fix = (g >> 31) + (g >= ng ? 1 : 0);
+                p += fix;
+                g -= fix * ng;
+                const bool ok = j < items;
+                const int pp = ok ? p : 0;
+                const int c1 = 2 * pp, c2 = 2 * pp + 1;
+                const uint32_t e1 = (uint32_t)(t0 + dcur[c1] + 4 * g);
+                const bool two = ch.c0 + c2 < nch;
+                const uint32_t e2 = e1 + (uint32_t)(two ? dcur[c2] : 0);
+                va[m] = ok ? 4u * ((uint32_t)c1 * ldw + e1 / EPD) : 0x80000000u;
+                vb[m] = ok && two ? 4u * ((uint32_t)c2 * ldw + e2 / EPD) : 0x80000000u;
+                pha[m] = (int)(e1 % EPD);
+                phb[m] = (int)(e2 % EPD);
+                wo[m] = ok ? pp * rowb + 16 * g : -1;
+                sa[m] = 1.0f;
+                sb[m] = 1.0f;
+                if constexpr (ESZ == 1) {
+                    sa[m] = chan_scale[ch.c0 + c1];
+                    sb[m] = two ? chan_scale[ch.c0 + c2] : 0.0f;
+                }
+            }
+            float x1[NI][4], x2[NI][K2];
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                load_elems<TS, 4>(rsrc, va[m], pha[m], sa[m], x1[m]);
+                load_elems<TS, K2>(rsrc, vb[m], phb[m], sb[m], x2[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                if (wo[m] < 0) continue;
+#pragma unroll
+                for (int i = 0; i < kPairMaxND; i++) {
+                    if (i >= nd) break;
+                    float4 w;
+                    w.x = x1[m][0] + x2[m][i];
+                    w.y = x1[m][1] + x2[m][i + 1];
+                    w.z = x1[m][2] + x2[m][i + 2];
+                    w.w = x1[m][3] + x2[m][i + 3];
+                    *reinterpret_cast<float4*>(data + wo[m] + i * ch.stride * 4) = w;
+                }
+            }
+        };
+        stage(0);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int i = tid + NT * r;
+            if (i < np * DMB / 4) reinterpret_cast<u32x4*>(roff)[i] = rv[r];
+        }
+        if constexpr (PairStageItems<EPD, W>::need > PairStageItems<EPD, W>::value) {
+            for (int j0 = NT * NI; j0 < items; j0 += NT * NI) stage(j0);
+        }
+        if (more && tid < 2 * nx.count && nx.c0 + tid < nch) dbuf[((k + 1) & 1) * kChunkMax + tid] = dnext;
+        __syncthreads();
+        for (int pi = 0; pi < np; pi++) {
+            const uint32_t* ro = roff + pi * DMB + wave * DW;
+#pragma unroll
+            for (int d = 0; d < DW; d++) {
+                const float* src = reinterpret_cast<const float*>(data + ro[d]) + lane;
+#pragma unroll
+                for (int q = 0; q < SL; q++) acc[d][q] += src[64 * q];
+            }
+        }
+        __syncthreads();
+    }
+    // lane element q is sample t0 + lane + 64q
+    const float fnch = (float)nch;
+    float mul = 1.0f, add = 0.0f;
+    if constexpr (ESZ == 2) mul = ldexpf(1.0f, f16_exp[0]);
+    if constexpr (ESZ == 1) add = offsum[0];
+#pragma unroll
+    for (int d = 0; d < DW; d++) {
+        const int dm = dm0 + d;
+        if (dm >= ndm) break;
+        float* o = out + (int64_t)dm * ld_out;
+#pragma unroll
+        for (int q = 0; q < SL; q++) {
+            const int t = t0 + lane + 64 * q;
+            const float a = acc[d][q];
+            if (t < nt) {
+                if constexpr (ESZ == 4) o[t] = a / fnch;
+                else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
+                else o[t] = (a + add) / fnch;
+            }
+        }
+    }
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
 // Remap the 1-D grid so that every XCD gets a contiguous range of tile ids
 // (ids 0..n-1 in XCD-major order), and order the ids DM-block fastest: the
@@ -457,7 +688,7 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
 }
 
 // One workgroup per tile: 1-D grid of nblk * (time tiles) workgroups.
-template <typename TS, int SL, int DW, int W>
+template <typename TS, int SL, int DW, int W, bool PAIR>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
               const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
@@ -467,8 +698,12 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int id = xcd_tile(blockIdx.x, gridDim.x);
     const int b = id % nblk, tx = id / nblk;
-    dedisp_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
-                               chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
+    if constexpr (PAIR)
+        dedisp_pair_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
+                                        chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
+    else
+        dedisp_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
+                                   chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
 }
 
 // ---------------------------------------------------------------------------
@@ -498,7 +733,7 @@ struct BatchTile {
 // ctr[0]: next tile, ctr[1]: workgroups done.  The last workgroup to finish
 // puts both back to zero, so the next launch (or graph replay) starts clean
 // without a memset; ntiles[0] = tiles in the list.
-template <typename TS, int SL, int DW, int W>
+template <typename TS, int SL, int DW, int W, bool PAIR>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restrict__ tiles,
                     const int* __restrict__ ntiles, unsigned* __restrict__ ctr) {
@@ -513,9 +748,14 @@ dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restri
         if (i >= total) break;
         const BatchTile tl = tiles[i];
         const BatchJob& j = jobs[tl.job];
-        dedisp_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
-                                   j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
-                                   j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
+        if constexpr (PAIR)
+            dedisp_pair_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
+                                            j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
+                                            j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
+        else
+            dedisp_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
+                                       j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
+                                       j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -554,15 +794,21 @@ __global__ void dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, i
 // ---------------------------------------------------------------------------
 struct TileCfg {
     int sl, dw, w;
+    bool pair;  // channel-pair presum tile
     const char* name;
 };
 // samples per lane (multiple of 4 so u8 lanes read whole dwords), DMs per
-// wave, waves per workgroup.  Tile = 64*SL samples x W*DW DMs.
+// wave, waves per workgroup.  Tile = 64*SL samples x W*DW DMs.  "P" tiles sum
+// channel pairs first.
 constexpr TileCfg kTiles[] = {
-    {4, 16, 4, "T256_dm64_w4"},   {8, 16, 4, "T512_dm64_w4"},  {8, 8, 4, "T512_dm32_w4"},
-    {8, 8, 8, "T512_dm64_w8"},    {4, 8, 8, "T256_dm64_w8"},   {8, 4, 8, "T512_dm32_w8"},
-    {16, 4, 8, "T1024_dm32_w8"},  {4, 16, 8, "T256_dm128_w8"}, {8, 4, 16, "T512_dm64_w16"},
-    {4, 4, 16, "T256_dm64_w16"},  {4, 8, 16, "T256_dm128_w16"}, {16, 8, 4, "T1024_dm32_w4"},
+    {4, 16, 4, false, "T256_dm64_w4"},    {8, 16, 4, false, "T512_dm64_w4"},
+    {8, 8, 4, false, "T512_dm32_w4"},     {8, 8, 8, false, "T512_dm64_w8"},
+    {4, 8, 8, false, "T256_dm64_w8"},     {8, 4, 8, false, "T512_dm32_w8"},
+    {16, 4, 8, false, "T1024_dm32_w8"},   {4, 16, 8, false, "T256_dm128_w8"},
+    {8, 4, 16, false, "T512_dm64_w16"},   {4, 4, 16, false, "T256_dm64_w16"},
+    {4, 8, 16, false, "T256_dm128_w16"},  {16, 8, 4, false, "T1024_dm32_w4"},
+    {8, 4, 16, true, "P512_dm64_w16"},    {8, 8, 8, true, "P512_dm64_w8"},
+    {4, 8, 16, true, "P256_dm128_w16"},   {16, 4, 8, true, "P1024_dm32_w8"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 constexpr int kDefaultTile = 3;
@@ -570,32 +816,23 @@ constexpr int kDefaultTile = 3;
 template <typename TS, int I>
 const void* tile_kernel_ptr(bool batch) {
     constexpr TileCfg c = kTiles[I];
-    return batch ? reinterpret_cast<const void*>(&dedisp_batch_kernel<TS, c.sl, c.dw, c.w>)
-                 : reinterpret_cast<const void*>(&dedisp_kernel<TS, c.sl, c.dw, c.w>);
+    return batch ? reinterpret_cast<const void*>(&dedisp_batch_kernel<TS, c.sl, c.dw, c.w, c.pair>)
+                 : reinterpret_cast<const void*>(&dedisp_kernel<TS, c.sl, c.dw, c.w, c.pair>);
+}
+template <typename TS, size_t... I>
+const void* tile_kernel_at(int i, bool batch, std::index_sequence<I...>) {
+    const void* k[] = {tile_kernel_ptr<TS, (int)I>(batch)...};
+    return k[i];
 }
 template <typename TS>
 const void* tile_kernel(int i, bool batch) {
-    switch (i) {
-        case 0: return tile_kernel_ptr<TS, 0>(batch);
-        case 1: return tile_kernel_ptr<TS, 1>(batch);
-        case 2: return tile_kernel_ptr<TS, 2>(batch);
-        case 3: return tile_kernel_ptr<TS, 3>(batch);
-        case 4: return tile_kernel_ptr<TS, 4>(batch);
-        case 5: return tile_kernel_ptr<TS, 5>(batch);
-        case 6: return tile_kernel_ptr<TS, 6>(batch);
-        case 7: return tile_kernel_ptr<TS, 7>(batch);
-        case 8: return tile_kernel_ptr<TS, 8>(batch);
-        case 9: return tile_kernel_ptr<TS, 9>(batch);
-        case 10: return tile_kernel_ptr<TS, 10>(batch);
-        default: return tile_kernel_ptr<TS, 11>(batch);
-    }
+    return tile_kernel_at<TS>(i, batch, std::make_index_sequence<kNumTiles>{});
 }
 const void* tile_kernel_for(int storage, int i, bool batch) {
     if (storage == VFRB_STORE_F32) return tile_kernel<float>(i, batch);
     if (storage == VFRB_STORE_F16) return tile_kernel<__half>(i, batch);
     return tile_kernel<uint8_t>(i, batch);
 }
-static_assert(kNumTiles == 12, "tile_kernel() covers 12 configurations");
 
 size_t storage_bytes(int storage) {
     return storage == VFRB_STORE_F32 ? 4 : storage == VFRB_STORE_F16 ? 2 : 1;
@@ -707,6 +944,70 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
             span[c] = hi - lo;
         }
         blk[b].x = (int)chunks.size();
+        if (tc.pair) {
+            // Pair p = channels 2p, 2p+1 (the last alone when nch is odd).
+            // dmin keeps the first channel's start row; the second channel's
+            // slot holds the pair's lowest relative delay dlo.  A chunk's
+            // pairs share the row count nd and the stride, a multiple of 4
+            // f32 samples covering T + span.
+            const int npair = (nch + 1) / 2;
+            std::vector<int> dlo(npair, 0), nd(npair, 1);
+            for (int pp = 0; pp < npair; pp++) {
+                const int c1 = 2 * pp, c2 = c1 + 1;
+                if (c2 >= nch) continue;
+                int lo = INT32_MAX, hi = INT32_MIN;
+                for (int d = d0; d < d1; d++) {
+                    const int dl = p->off[(size_t)d * nch + c2] - p->off[(size_t)d * nch + c1];
+                    lo = std::min(lo, dl);
+                    hi = std::max(hi, dl);
+                }
+                dlo[pp] = lo;
+                nd[pp] = hi - lo + 1;
+                VFRB_REQUIRE(nd[pp] <= kPairMaxND, VFRB_ERR_UNSUPPORTED,
+                             "pair tile: the relative delay of channels %d,%d spans %d rows across a DM "
+                             "block (at most %d)", c1, c2, nd[pp], kPairMaxND);
+            }
+            for (int pp = 0; pp < npair; pp++)
+                if (2 * pp + 1 < nch) dmin[(size_t)b * nch + 2 * pp + 1] = dlo[pp];
+            auto pstride = [&](int sp) { return (T + sp + 3) / 4 * 4; };
+            int pp = 0;
+            while (pp < npair) {
+                int count = 0, smax = 0, nmax = 1;
+                while (pp + count < npair && count < kPairMax) {
+                    const int s2 = std::max(smax, span[2 * (pp + count)]);
+                    const int n2 = std::max(nmax, nd[pp + count]);
+                    if ((size_t)(count + 1) * n2 * 4 * (size_t)pstride(s2) > kDataBudget) break;
+                    smax = s2;
+                    nmax = n2;
+                    count++;
+                }
+                VFRB_REQUIRE(count > 0, VFRB_ERR_UNSUPPORTED,
+                             "one channel pair's presum rows exceed the LDS budget; use a tile with fewer "
+                             "DMs per block");
+                const int stride = pstride(smax);
+                chunks.push_back(Chunk{2 * pp, count, stride, (int)roffs.size(), nmax, {0, 0, 0}});
+                for (int pi = 0; pi < count; pi++) {
+                    const int c1 = 2 * (pp + pi), c2 = c1 + 1;
+                    for (int d = 0; d < DMB; d++) {
+                        uint32_t o = 0;
+                        if (d0 + d < d1) {
+                            const int a1 = p->off[(size_t)(d0 + d) * nch + c1] - dmin[(size_t)b * nch + c1];
+                            const int row = c2 < nch ? p->off[(size_t)(d0 + d) * nch + c2] -
+                                                           p->off[(size_t)(d0 + d) * nch + c1] - dlo[pp + pi]
+                                                     : 0;
+                            o = (uint32_t)(((pi * nmax + row) * stride + a1) * 4);
+                        }
+                        roffs.push_back(o);
+                    }
+                }
+                max_data = std::max(max_data, (size_t)count * nmax * 4 * stride);
+                // accumulate (one add per pair) + staging (nd presum rows per pair)
+                cost[b] += (int64_t)count * ((int64_t)DMB * T + 2LL * stride * nmax);
+                pp += count;
+            }
+            blk[b].y = (int)chunks.size() - blk[b].x;
+            continue;
+        }
         int c = 0;
         while (c < nch) {
             int count = 0, smax = 0;
@@ -721,7 +1022,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
                          "one channel's staged span (%d rows) exceeds the LDS budget; use a tile "
                          "with fewer DMs per block", T + span[c]);
             const int stride = stride_of(smax);
-            chunks.push_back(Chunk{c, count, stride, (int)roffs.size()});
+            chunks.push_back(Chunk{c, count, stride, (int)roffs.size(), 1, {0, 0, 0}});
             for (int ci = 0; ci < count; ci++) {
                 const int cc = c + ci;
                 const int phase = dmin[(size_t)b * nch + cc] & (epd - 1);
