@@ -314,3 +314,34 @@ def test_candidates(gpu):
     snr = (x - st.ave[:, None]) / st.sigma[:, None]
     want = set(zip(*np.nonzero(snr > 7.0)))
     assert n == len(want) and set(zip(cdm.tolist(), ct.tolist())) == want
+
+
+def test_default_tile_falls_back_for_coarse_dm_lists(gpu, orc):
+    """A DM list coarse enough that adjacent channels' relative delay varies
+    by more than 4 rows across a 64-DM block (DM step 5 at 1024 ch) cannot
+    use the channel-pair tile: a default plan, and a default batch, fall
+    back to a per-channel tile with the same results."""
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, _ = _c1_shape()
+    dms = (np.arange(64) * 5.0).astype(np.float32)
+    rows = 4200
+    rng = np.random.default_rng(41)
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    print("[case] default tile fallback", flush=True)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
+    names = vfrb.tile_names()
+    assert names[plan.tile][0] == "T"
+    with pytest.raises(vfrb.VfrbError):
+        plan.set_tile(names.index("P512_dm64_w16"))
+    plan.set_tile(names.index("T512_dm64_w8"))
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, bandpass=False)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    batch = vfrb.Batch(vfrb.STORE_F32)
+    batch.set([(plan, rows, out, nt)])
+    assert names[batch.tile][0] == "T"
+    batch.dedisperse()
+    vfrb.sync()
+    ref = orc.dedisperse_f32(fb, plan.offsets(), nt)
+    assert np.max(np.abs(out.download() - ref)) < 1e-4 * 22

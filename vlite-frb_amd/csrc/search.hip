@@ -811,7 +811,11 @@ constexpr TileCfg kTiles[] = {
     {4, 8, 16, true, "P256_dm128_w16"},   {16, 4, 8, true, "P1024_dm32_w8"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
-constexpr int kDefaultTile = 3;
+// Default tile: the channel-pair tile, which needs the relative delay of
+// adjacent channels to vary by at most kPairMaxND rows across a DM block; for
+// DM lists too coarse for that, the per-channel kFallbackTile.
+constexpr int kDefaultTile = 12;  // P512_dm64_w16
+constexpr int kFallbackTile = 3;  // T512_dm64_w8
 
 template <typename TS, int I>
 const void* tile_kernel_ptr(bool batch) {
@@ -872,6 +876,7 @@ struct vfrb_search_plan {
 
 struct vfrb_batch {
     int storage = 0, tile = -1;
+    bool tile_default = false;  // created with tile_config -1: may fall back
     int max_jobs = 0, max_tiles = 0;
     int ntiles = -1;  // -1: no jobs set
     std::vector<vfrb_search_plan*> plans;
@@ -1146,6 +1151,7 @@ vfrb_status vfrb_search_plan_create(vfrb_search_plan** plan, const vfrb_search_d
         return st;
     }
     st = build_tiles(p, d->tile_config >= 0 ? d->tile_config : kDefaultTile);
+    if (st == VFRB_ERR_UNSUPPORTED && d->tile_config < 0) st = build_tiles(p, kFallbackTile);
     if (st != VFRB_OK) {
         destroy(p);
         return st;
@@ -1463,6 +1469,7 @@ vfrb_status vfrb_batch_create(vfrb_batch** batch, int32_t storage, int32_t tile_
         batch_free(b);
         return st;
     }
+    b->tile_default = tile_config < 0;
     const vfrb_status st = batch_configure(b, tile_config >= 0 ? tile_config : kDefaultTile);
     if (st != VFRB_OK) {
         batch_free(b);
@@ -1500,7 +1507,11 @@ vfrb_status vfrb_batch_set(vfrb_batch* b, int32_t njobs, vfrb_search_plan* const
     b->nrows.assign(nrows, nrows + njobs);
     b->outs.assign(outs, outs + njobs);
     b->ld_outs.assign(ld_outs, ld_outs + njobs);
-    const vfrb_status st = batch_build(b, vfrb::as_stream(stream));
+    vfrb_status st = batch_build(b, vfrb::as_stream(stream));
+    if (st == VFRB_ERR_UNSUPPORTED && b->tile_default && b->tile == kDefaultTile) {
+        st = batch_configure(b, kFallbackTile);
+        if (st == VFRB_OK) st = batch_build(b, vfrb::as_stream(stream));
+    }
     if (st != VFRB_OK) {
         b->ntiles = -1;
         return st;
