@@ -209,6 +209,26 @@ struct Chunk {
     int pad[3];
 };
 
+// Tables read through pointers of unknown address space (the batch kernel's
+// job records) load with flat vector loads; these helpers put the uniform
+// values back into scalar registers so that loop bounds and strides stay
+// uniform.
+__device__ __forceinline__ int2 uniform_load(const int2* p) {
+    const int2 v = *p;
+    return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
+}
+__device__ __forceinline__ Chunk uniform_load(const Chunk* p) {
+    const Chunk v = *p;
+    Chunk u;
+    u.c0 = __builtin_amdgcn_readfirstlane(v.c0);
+    u.count = __builtin_amdgcn_readfirstlane(v.count);
+    u.stride = __builtin_amdgcn_readfirstlane(v.stride);
+    u.roff = __builtin_amdgcn_readfirstlane(v.roff);
+    u.nd = __builtin_amdgcn_readfirstlane(v.nd);
+    u.pad[0] = u.pad[1] = u.pad[2] = 0;
+    return u;
+}
+
 // At most kChunkMax channels per LDS chunk; staged rows take at most
 // kDataBudget bytes, so with the offset table a workgroup stays under 80 KB
 // and two fit in one CU's 160 KB of LDS.
@@ -281,7 +301,7 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int t0 = tx * T;
     const int dm0 = b * DMB + wave * DW;
-    const int2 cr = blk_chunks[b];
+    const int2 cr = uniform_load(blk_chunks + b);
     const int* dmin_b = dmin + (int64_t)b * nch;
     const uint32_t ldw = (uint32_t)(ld / EPD);  // dwords per window row
 
@@ -293,13 +313,13 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
 
     // first chunk's channel start rows
     if (cr.y > 0) {
-        const Chunk c0 = chunks[cr.x];
+        const Chunk c0 = uniform_load(chunks + cr.x);
         if (tid < c0.count) dbuf[tid] = dmin_b[c0.c0 + tid];
     }
     __syncthreads();
 
     for (int k = 0; k < cr.y; k++) {
-        const Chunk ch = chunks[cr.x + k];
+        const Chunk ch = uniform_load(chunks + cr.x + k);
         const int rowb = 4 * ch.stride;    // bytes per staged channel (all copies)
         const int ng = ch.stride / EPD / 4;  // 4-dword groups per copy
         const int items = ch.count * ng;
@@ -319,7 +339,7 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
         const bool more = k + 1 < cr.y;
         Chunk nx = ch;
         if (more) {
-            nx = chunks[cr.x + k + 1];
+            nx = uniform_load(chunks + cr.x + k + 1);
             if (tid < nx.count) dnext = dmin_b[nx.c0 + tid];
         }
         // Item j = (channel ci, group g), ci = j / ng: a float quotient, off by
@@ -531,7 +551,7 @@ dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int t0 = tx * T;
     const int dm0 = b * DMB + wave * DW;
-    const int2 cr = blk_chunks[b];
+    const int2 cr = uniform_load(blk_chunks + b);
     const int* dmin_b = dmin + (int64_t)b * nch;
     const uint32_t ldw = (uint32_t)(ld / EPD);
 
@@ -543,29 +563,23 @@ dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int
 
     // dbuf[parity][2p] = start row of pair p's first channel, [2p+1] = dlo
     if (cr.y > 0) {
-        const Chunk c0 = chunks[cr.x];
+        const Chunk c0 = uniform_load(chunks + cr.x);
         if (tid < 2 * c0.count && c0.c0 + tid < nch) dbuf[tid] = dmin_b[c0.c0 + tid];
     }
     __syncthreads();
 
     for (int k = 0; k < cr.y; k++) {
-        const Chunk ch = chunks[cr.x + k];
+        const Chunk ch = uniform_load(chunks + cr.x + k);
         const int np = ch.count, nd = ch.nd;
         const int rowb = nd * ch.stride * 4;  // bytes per pair (nd presum rows)
         const int ng = ch.stride / 4;         // 4-sample groups per row
         const int items = np * ng;
         const int* dcur = dbuf + (k & 1) * kChunkMax;
-        u32x4 rv[NR];
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int i = tid + NT * r;
-            if (i < np * DMB / 4) rv[r] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
-        }
         int dnext = 0;
         const bool more = k + 1 < cr.y;
         Chunk nx = ch;
         if (more) {
-            nx = chunks[cr.x + k + 1];
+            nx = uniform_load(chunks + cr.x + k + 1);
             if (tid < 2 * nx.count && nx.c0 + tid < nch) dnext = dmin_b[nx.c0 + tid];
         }
         const TS* base = win + (int64_t)ch.c0 * ld;
@@ -630,10 +644,13 @@ dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int
             }
         };
         stage(0);
+        // the chunk's offset table, loaded after the staged groups so that
+        // its registers are not live while they are
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int i = tid + NT * r;
-            if (i < np * DMB / 4) reinterpret_cast<u32x4*>(roff)[i] = rv[r];
+            if (i < np * DMB / 4)
+                reinterpret_cast<u32x4*>(roff)[i] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
         }
         if constexpr (PairStageItems<EPD, W>::need > PairStageItems<EPD, W>::value) {
             for (int j0 = NT * NI; j0 < items; j0 += NT * NI) stage(j0);
