@@ -125,7 +125,7 @@ def test_delay_table_device_matches_oracle(gpu, orc):
     assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
 
 
-@pytest.mark.parametrize("tile", range(16))
+@pytest.mark.parametrize("tile", range(20))
 def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()
@@ -146,8 +146,9 @@ def test_dedisperse_f32_all_tiles(gpu, orc, tile):
 
 @pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
 def test_pair_tiles_match_channel_tiles(gpu, storage):
-    """The pair-presum tiles add the same terms as the per-channel tiles, each
-    channel pair first: equal to f32 rounding on the same stored window.  The
+    """The pair- and group-presum tiles add the same terms as the per-channel
+    tiles, each channel pair (group of 4) first: equal to f32 rounding on the
+    same stored window.  The
     u8 accumulator holds sum(scale*byte), of the size of the channel offsets
     subtracted afterwards (about 60x the output here), so its summation-order
     rounding is that much larger relative to the output."""
@@ -164,7 +165,7 @@ def test_pair_tiles_match_channel_tiles(gpu, storage):
     plan.dedisperse(rows, out, nt)
     ref = out.download()
     names = vfrb.tile_names()
-    for i in (i for i, n in enumerate(names) if n.startswith("P")):
+    for i in (i for i, n in enumerate(names) if n[0] in "PG"):
         print(f"[case] pair tile {names[i]} {storage}", flush=True)
         plan.set_tile(i)
         out.zero()

@@ -10,6 +10,7 @@
 // for each chunk of channels it stages window[c][t0+dmin_c .. t0+T+dmax_c)
 // in LDS once and every wave accumulates its DMs from there.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <utility>
@@ -205,8 +206,8 @@ constexpr h2_t kSelHi = {(_Float16)0.0f, (_Float16)1.0f};
 struct Chunk {
     int c0, count, stride;  // stride: staged samples per copy, multiple of 4
     int roff;               // first entry of this chunk in the LDS byte-offset table
-    int nd;                 // pair tiles: presum rows per channel pair (else 1)
-    int pad[3];
+    int nd;                 // pair tiles: presum rows per channel pair; group tiles: rows in the chunk
+    int pad[3];             // group tiles: pad[0] = first int of the chunk's row records
 };
 
 // Tables read through pointers of unknown address space (the batch kernel's
@@ -225,7 +226,8 @@ __device__ __forceinline__ Chunk uniform_load(const Chunk* p) {
     u.stride = __builtin_amdgcn_readfirstlane(v.stride);
     u.roff = __builtin_amdgcn_readfirstlane(v.roff);
     u.nd = __builtin_amdgcn_readfirstlane(v.nd);
-    u.pad[0] = u.pad[1] = u.pad[2] = 0;
+    u.pad[0] = __builtin_amdgcn_readfirstlane(v.pad[0]);
+    u.pad[1] = u.pad[2] = 0;
     return u;
 }
 
@@ -236,11 +238,18 @@ constexpr int kChunkMax = 32;
 constexpr size_t kDataBudget = 60 * 1024;
 constexpr int kBufferFlags = 0x00020000;  // word 3 of a gfx9 raw buffer resource
 
+// Group tiles: presum rows per chunk at most, and the record of each row
+// (8 ints: channel of its group within the chunk, the group's start row, the
+// three relative delays of the group's other channels, its channel count).
+constexpr int kRowMax = 48;
+constexpr int kRowInts = 8;
+
 // LDS ahead of the staged rows: the chunk's [kChunkMax][DMB] u32 read offsets,
-// then two [kChunkMax] buffers of channel start rows (dmin), one per chunk
-// parity, so that chunk k+1's are loaded while chunk k is staged.
+// then two buffers, one per chunk parity, of the chunk's channel start rows
+// (dmin; group tiles: its row records), so that chunk k+1's are loaded while
+// chunk k is staged.
 __host__ __device__ constexpr size_t lds_head_bytes(int dmb) {
-    return (size_t)kChunkMax * dmb * 4 + 2 * kChunkMax * 4;
+    return (size_t)kChunkMax * dmb * 4 + 2 * kRowMax * kRowInts * 4;
 }
 
 // Staging items (channel, 4-dword group) each thread loads per pass: enough
@@ -691,6 +700,188 @@ dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int
     }
 }
 
+// ---------------------------------------------------------------------------
+// Group tiles (4 channels).  The DMs of a block see a group's channels at a
+// few distinct delay tuples (off(d,c+k) - off(d,c), k = 1..3; about 4 per
+// group of 4 at 4096 ch x 2048 DM).  Each distinct tuple is one f32 presum
+// row, R_r[s] = sum_k x[c+k][S + s + d_k], and every DM adds one presum per
+// group: a quarter of the per-channel adds and half the pair tile's LDS reads
+// per sample, for the same staged bytes (4 B per channel-sample).  The host
+// lists a chunk's rows as records in the dmin array.
+// ---------------------------------------------------------------------------
+constexpr int kGroupCh = 4;
+constexpr int kGroupMax = kChunkMax / kGroupCh;
+
+template <int W>
+struct GroupStageItems {
+    static constexpr int groups = (int)(kDataBudget / 16);  // 16 B per item
+    static constexpr int need = (groups + 64 * W - 1) / (64 * W);
+    static constexpr int regs = kGroupCh * 4 + 4;
+    static constexpr int fit = (1024 / W / 4) / regs;
+    static constexpr int cap = fit < 1 ? 1 : (fit > 8 ? 8 : fit);
+    static constexpr int value = need < cap ? need : cap;
+};
+
+template <typename TS, int SL, int DW, int W>
+__device__ __forceinline__ void
+dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+                  const uint32_t* __restrict__ roffs,  // per chunk: [groups][W*DW] LDS byte offsets
+                  const int* __restrict__ recs_g,      // row records, kRowInts ints each
+                  const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+                  const float* __restrict__ chan_scale, const int* __restrict__ f16_exp,
+                  const float* __restrict__ offsum, float* __restrict__ out, int64_t ld_out, int b, int tx,
+                  unsigned char* lds_raw) {
+    constexpr int ESZ = (int)sizeof(TS);
+    constexpr int EPD = 4 / ESZ;
+    constexpr int T = 64 * SL;
+    constexpr int DMB = W * DW;
+    constexpr int NT = 64 * W;
+    constexpr int NI = GroupStageItems<W>::value;
+    constexpr int NR = (kGroupMax * DMB / 4 + NT - 1) / NT;
+    constexpr int NQ = (kRowMax * kRowInts + NT - 1) / NT;  // record ints per thread
+    uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
+    int* rbuf = reinterpret_cast<int*>(lds_raw + (size_t)kChunkMax * DMB * 4);
+    unsigned char* data = lds_raw + lds_head_bytes(DMB);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int t0 = tx * T;
+    const int dm0 = b * DMB + wave * DW;
+    const int2 cr = uniform_load(blk_chunks + b);
+    const uint32_t ldw = (uint32_t)(ld / EPD);
+
+    float acc[DW][SL];
+#pragma unroll
+    for (int d = 0; d < DW; d++)
+#pragma unroll
+        for (int j = 0; j < SL; j++) acc[d][j] = 0.0f;
+
+    if (cr.y > 0) {
+        const Chunk c0 = uniform_load(chunks + cr.x);
+        for (int i = tid; i < c0.nd * kRowInts; i += NT) rbuf[i] = recs_g[c0.pad[0] + i];
+    }
+    __syncthreads();
+
+    for (int k = 0; k < cr.y; k++) {
+        const Chunk ch = uniform_load(chunks + cr.x + k);
+        const int ngr = ch.count, nrw = ch.nd;
+        const int ng = ch.stride / 4;  // 4-sample groups per row
+        const int items = nrw * ng;
+        const int* rcur = rbuf + (k & 1) * kRowMax * kRowInts;
+        const bool more = k + 1 < cr.y;
+        Chunk nx = ch;
+        int qn[NQ];
+        if (more) {
+            nx = uniform_load(chunks + cr.x + k + 1);
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int i = tid + NT * q;
+                qn[q] = i < nx.nd * kRowInts ? recs_g[nx.pad[0] + i] : 0;
+            }
+        }
+        const TS* base = win + (int64_t)ch.c0 * ld;
+        const int64_t left = ((int64_t)nch - ch.c0) * ld * ESZ;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<TS*>(base), 0, (int)(left < 0x7fffffff ? left : 0x7fffffff), kBufferFlags);
+        const float rng = 1.0f / (float)ng;
+        // Item j = (row r, 4-sample group g): the row's four channels' elements
+        // S + d_k + 4g .. +3 (buffer loads; zeros past the window or for a
+        // channel the last group lacks) summed into presum samples 4g..4g+3.
+        auto stage = [&](int j0) {
+            uint32_t vo[NI][kGroupCh];
+            int ph[NI][kGroupCh], wo[NI];
+            float sc[NI][kGroupCh];
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                const int j = j0 + tid + NT * m;
+                int r = (int)((float)j * rng);
+                int g = j - r * ng;
+                const int fix = (g >> 31) + (g >= ng ? 1 : 0);
+                r += fix;
+                g -= fix * ng;
+                const bool ok = j < items;
+                const int* rec = rcur + (ok ? r : 0) * kRowInts;
+                const int crel = rec[0], nchg = rec[5];
+                const uint32_t e0 = (uint32_t)(t0 + rec[1] + 4 * g);
+#pragma unroll
+                for (int c = 0; c < kGroupCh; c++) {
+                    const uint32_t e = e0 + (uint32_t)(c == 0 ? 0 : rec[1 + c]);
+                    vo[m][c] = ok && c < nchg ? 4u * ((uint32_t)(crel + c) * ldw + e / EPD) : 0x80000000u;
+                    ph[m][c] = (int)(e % EPD);
+                    sc[m][c] = 1.0f;
+                    if constexpr (ESZ == 1) sc[m][c] = c < nchg ? chan_scale[ch.c0 + crel + c] : 0.0f;
+                }
+                wo[m] = ok ? (r * ch.stride + 4 * g) * 4 : -1;
+            }
+            float x[NI][kGroupCh][4];
+#pragma unroll
+            for (int m = 0; m < NI; m++)
+#pragma unroll
+                for (int c = 0; c < kGroupCh; c++) load_elems<TS, 4>(rsrc, vo[m][c], ph[m][c], sc[m][c], x[m][c]);
+#pragma unroll
+            for (int m = 0; m < NI; m++) {
+                if (wo[m] < 0) continue;
+                float4 w;
+                w.x = (x[m][0][0] + x[m][1][0]) + (x[m][2][0] + x[m][3][0]);
+                w.y = (x[m][0][1] + x[m][1][1]) + (x[m][2][1] + x[m][3][1]);
+                w.z = (x[m][0][2] + x[m][1][2]) + (x[m][2][2] + x[m][3][2]);
+                w.w = (x[m][0][3] + x[m][1][3]) + (x[m][2][3] + x[m][3][3]);
+                *reinterpret_cast<float4*>(data + wo[m]) = w;
+            }
+        };
+        stage(0);
+        if constexpr (GroupStageItems<W>::need > GroupStageItems<W>::value) {
+            for (int j0 = NT * NI; j0 < items; j0 += NT * NI) stage(j0);
+        }
+        // the chunk's offset table, and the next chunk's row records
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int i = tid + NT * r;
+            if (i < ngr * DMB / 4)
+                reinterpret_cast<u32x4*>(roff)[i] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
+        }
+        if (more) {
+            int* rnext = rbuf + ((k + 1) & 1) * kRowMax * kRowInts;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int i = tid + NT * q;
+                if (i < nx.nd * kRowInts) rnext[i] = qn[q];
+            }
+        }
+        __syncthreads();
+        for (int gi = 0; gi < ngr; gi++) {
+            const uint32_t* ro = roff + gi * DMB + wave * DW;
+#pragma unroll
+            for (int d = 0; d < DW; d++) {
+                const float* src = reinterpret_cast<const float*>(data + ro[d]) + lane;
+#pragma unroll
+                for (int q = 0; q < SL; q++) acc[d][q] += src[64 * q];
+            }
+        }
+        __syncthreads();
+    }
+    const float fnch = (float)nch;
+    float mul = 1.0f, add = 0.0f;
+    if constexpr (ESZ == 2) mul = ldexpf(1.0f, f16_exp[0]);
+    if constexpr (ESZ == 1) add = offsum[0];
+#pragma unroll
+    for (int d = 0; d < DW; d++) {
+        const int dm = dm0 + d;
+        if (dm >= ndm) break;
+        float* o = out + (int64_t)dm * ld_out;
+#pragma unroll
+        for (int q = 0; q < SL; q++) {
+            const int t = t0 + lane + 64 * q;
+            const float a = acc[d][q];
+            if (t < nt) {
+                if constexpr (ESZ == 4) o[t] = a / fnch;
+                else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
+                else o[t] = (a + add) / fnch;
+            }
+        }
+    }
+}
+
 // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
 // Remap the 1-D grid so that every XCD gets a contiguous range of tile ids
 // (ids 0..n-1 in XCD-major order), and order the ids DM-block fastest: the
@@ -704,8 +895,9 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
     return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
 }
 
-// One workgroup per tile: 1-D grid of nblk * (time tiles) workgroups.
-template <typename TS, int SL, int DW, int W, bool PAIR>
+// One workgroup per tile: 1-D grid of nblk * (time tiles) workgroups.  MODE
+// 0: per-channel tile; 1: channel-pair presums; 2: 4-channel group presums.
+template <typename TS, int SL, int DW, int W, int MODE>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
               const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
@@ -715,7 +907,10 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int id = xcd_tile(blockIdx.x, gridDim.x);
     const int b = id % nblk, tx = id / nblk;
-    if constexpr (PAIR)
+    if constexpr (MODE == 2)
+        dedisp_group_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
+                                         chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
+    else if constexpr (MODE == 1)
         dedisp_pair_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
                                         chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
     else
@@ -750,7 +945,7 @@ struct BatchTile {
 // ctr[0]: next tile, ctr[1]: workgroups done.  The last workgroup to finish
 // puts both back to zero, so the next launch (or graph replay) starts clean
 // without a memset; ntiles[0] = tiles in the list.
-template <typename TS, int SL, int DW, int W, bool PAIR>
+template <typename TS, int SL, int DW, int W, int MODE>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
 dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restrict__ tiles,
                     const int* __restrict__ ntiles, unsigned* __restrict__ ctr) {
@@ -765,7 +960,11 @@ dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restri
         if (i >= total) break;
         const BatchTile tl = tiles[i];
         const BatchJob& j = jobs[tl.job];
-        if constexpr (PAIR)
+        if constexpr (MODE == 2)
+            dedisp_group_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
+                                             j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
+                                             j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
+        else if constexpr (MODE == 1)
             dedisp_pair_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
                                             j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
                                             j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
@@ -811,21 +1010,23 @@ __global__ void dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, i
 // ---------------------------------------------------------------------------
 struct TileCfg {
     int sl, dw, w;
-    bool pair;  // channel-pair presum tile
+    int mode;  // 0 per channel; 1 channel-pair presums ("P"); 2 4-channel group presums ("G")
     const char* name;
 };
 // samples per lane (multiple of 4 so u8 lanes read whole dwords), DMs per
 // wave, waves per workgroup.  Tile = 64*SL samples x W*DW DMs.  "P" tiles sum
-// channel pairs first.
+// channel pairs first, "G" tiles groups of 4 channels.
 constexpr TileCfg kTiles[] = {
-    {4, 16, 4, false, "T256_dm64_w4"},    {8, 16, 4, false, "T512_dm64_w4"},
-    {8, 8, 4, false, "T512_dm32_w4"},     {8, 8, 8, false, "T512_dm64_w8"},
-    {4, 8, 8, false, "T256_dm64_w8"},     {8, 4, 8, false, "T512_dm32_w8"},
-    {16, 4, 8, false, "T1024_dm32_w8"},   {4, 16, 8, false, "T256_dm128_w8"},
-    {8, 4, 16, false, "T512_dm64_w16"},   {4, 4, 16, false, "T256_dm64_w16"},
-    {4, 8, 16, false, "T256_dm128_w16"},  {16, 8, 4, false, "T1024_dm32_w4"},
-    {8, 4, 16, true, "P512_dm64_w16"},    {8, 8, 8, true, "P512_dm64_w8"},
-    {4, 8, 16, true, "P256_dm128_w16"},   {16, 4, 8, true, "P1024_dm32_w8"},
+    {4, 16, 4, 0, "T256_dm64_w4"},    {8, 16, 4, 0, "T512_dm64_w4"},
+    {8, 8, 4, 0, "T512_dm32_w4"},     {8, 8, 8, 0, "T512_dm64_w8"},
+    {4, 8, 8, 0, "T256_dm64_w8"},     {8, 4, 8, 0, "T512_dm32_w8"},
+    {16, 4, 8, 0, "T1024_dm32_w8"},   {4, 16, 8, 0, "T256_dm128_w8"},
+    {8, 4, 16, 0, "T512_dm64_w16"},   {4, 4, 16, 0, "T256_dm64_w16"},
+    {4, 8, 16, 0, "T256_dm128_w16"},  {16, 8, 4, 0, "T1024_dm32_w4"},
+    {8, 4, 16, 1, "P512_dm64_w16"},   {8, 8, 8, 1, "P512_dm64_w8"},
+    {4, 8, 16, 1, "P256_dm128_w16"},  {16, 4, 8, 1, "P1024_dm32_w8"},
+    {8, 4, 16, 2, "G512_dm64_w16"},   {8, 8, 8, 2, "G512_dm64_w8"},
+    {4, 8, 16, 2, "G256_dm128_w16"},  {4, 4, 16, 2, "G256_dm64_w16"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 // Default tile: the channel-pair tile, which needs the relative delay of
@@ -837,8 +1038,8 @@ constexpr int kFallbackTile = 3;  // T512_dm64_w8
 template <typename TS, int I>
 const void* tile_kernel_ptr(bool batch) {
     constexpr TileCfg c = kTiles[I];
-    return batch ? reinterpret_cast<const void*>(&dedisp_batch_kernel<TS, c.sl, c.dw, c.w, c.pair>)
-                 : reinterpret_cast<const void*>(&dedisp_kernel<TS, c.sl, c.dw, c.w, c.pair>);
+    return batch ? reinterpret_cast<const void*>(&dedisp_batch_kernel<TS, c.sl, c.dw, c.w, c.mode>)
+                 : reinterpret_cast<const void*>(&dedisp_kernel<TS, c.sl, c.dw, c.w, c.mode>);
 }
 template <typename TS, size_t... I>
 const void* tile_kernel_at(int i, bool batch, std::index_sequence<I...>) {
@@ -943,6 +1144,7 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     const int epd = 4 / esz;
     size_t max_data = 0;
     std::vector<int> dmin((size_t)nblk * nch, 0);
+    std::vector<int> recs;  // group tiles: row records, uploaded in place of dmin
     std::vector<Chunk> chunks;
     std::vector<uint32_t> roffs;
     std::vector<int2> blk(nblk);
@@ -966,7 +1168,75 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
             span[c] = hi - lo;
         }
         blk[b].x = (int)chunks.size();
-        if (tc.pair) {
+        if (tc.mode == 2) {
+            // Group q = channels 4q..4q+3 (fewer in the last group).  Each
+            // distinct tuple of relative delays of the group's channels over
+            // the block's DMs is one presum row; a chunk's rows are listed as
+            // records (kRowInts ints) appended to recs.
+            const int ngrp = (nch + kGroupCh - 1) / kGroupCh;
+            std::vector<std::vector<std::array<int, 3>>> tup(ngrp);
+            std::vector<std::vector<int>> rowof(ngrp);
+            for (int g = 0; g < ngrp; g++) {
+                const int c0 = kGroupCh * g, nchg = std::min(kGroupCh, nch - c0);
+                for (int d = d0; d < d1; d++) {
+                    std::array<int, 3> t{0, 0, 0};
+                    for (int k = 1; k < nchg; k++)
+                        t[k - 1] = p->off[(size_t)d * nch + c0 + k] - p->off[(size_t)d * nch + c0];
+                    int idx = -1;
+                    for (int r = 0; r < (int)tup[g].size() && idx < 0; r++)
+                        if (tup[g][r] == t) idx = r;
+                    if (idx < 0) {
+                        idx = (int)tup[g].size();
+                        tup[g].push_back(t);
+                    }
+                    rowof[g].push_back(idx);
+                }
+            }
+            auto gstride = [&](int sp) { return (T + sp + 3) / 4 * 4; };
+            int g0 = 0;
+            while (g0 < ngrp) {
+                int count = 0, rows = 0, smax = 0;
+                while (g0 + count < ngrp && count < kGroupMax) {
+                    const int r2 = rows + (int)tup[g0 + count].size();
+                    const int s2 = std::max(smax, span[kGroupCh * (g0 + count)]);
+                    if (r2 > kRowMax || (size_t)r2 * 4 * (size_t)gstride(s2) > kDataBudget) break;
+                    rows = r2;
+                    smax = s2;
+                    count++;
+                }
+                VFRB_REQUIRE(count > 0, VFRB_ERR_UNSUPPORTED,
+                             "one channel group's presum rows exceed the LDS budget; use a tile with fewer DMs "
+                             "per block");
+                const int stride = gstride(smax);
+                chunks.push_back(Chunk{kGroupCh * g0, count, stride, (int)roffs.size(), rows,
+                                       {(int)recs.size(), 0, 0}});
+                int rowstart = 0;
+                for (int q = 0; q < count; q++) {
+                    const int g = g0 + q, c0 = kGroupCh * g, nchg = std::min(kGroupCh, nch - c0);
+                    const int amin = dmin[(size_t)b * nch + c0];
+                    for (const auto& t : tup[g]) {
+                        const int rec[kRowInts] = {kGroupCh * q, amin, t[0], t[1], t[2], nchg, 0, 0};
+                        recs.insert(recs.end(), rec, rec + kRowInts);
+                    }
+                    for (int d = 0; d < DMB; d++) {
+                        uint32_t o = 0;
+                        if (d0 + d < d1) {
+                            const int a = p->off[(size_t)(d0 + d) * nch + c0] - amin;
+                            o = (uint32_t)(((rowstart + rowof[g][d]) * stride + a) * 4);
+                        }
+                        roffs.push_back(o);
+                    }
+                    rowstart += (int)tup[g].size();
+                }
+                max_data = std::max(max_data, (size_t)rows * 4 * stride);
+                // accumulate (one add per group) + staging (one presum per row sample)
+                cost[b] += (int64_t)count * DMB * T + 4LL * stride * rows;
+                g0 += count;
+            }
+            blk[b].y = (int)chunks.size() - blk[b].x;
+            continue;
+        }
+        if (tc.mode == 1) {
             // Pair p = channels 2p, 2p+1 (the last alone when nch is odd).
             // dmin keeps the first channel's start row; the second channel's
             // slot holds the pair's lowest relative delay dlo.  A chunk's
@@ -1062,6 +1332,8 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
         }
         blk[b].y = (int)chunks.size() - blk[b].x;
     }
+    if (tc.mode == 2) dmin.swap(recs);
+    if (dmin.empty()) dmin.push_back(0);
     free_tiles(p);
     hipError_t e = hipMalloc(&p->d_roffs, roffs.size() * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMalloc(&p->d_dmin, dmin.size() * sizeof(int));
