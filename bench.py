@@ -316,10 +316,11 @@ def main():
     dom = max(st, key=lambda k: st[k])
     dur_s = st[dom] * 1e-3
     if dom == "dedisperse":
-        kind, sl, dmb, w = re.match(r"([TP])(\d+)_dm(\d+)_w(\d+)", main_r["tile"]).groups()
+        kind, sl, dmb, w = re.match(r"([TPG])(\d+)_dm(\d+)_w(\d+)", main_r["tile"]).groups()
         sl, dmb, w = int(sl), int(dmb), int(w)
         ts = {"f32": "float", "f16": "__half", "u8": "unsigned char"}[args.storage]
-        kname = f"dedisp_kernel<{ts}, {sl // 64}, {dmb // w}, {w}, {'true' if kind == 'P' else 'false'}>"
+        mode = "TPG".index(kind)  # per-channel, channel-pair, 4-channel-group tile
+        kname = f"dedisp_kernel<{ts}, {sl // 64}, {dmb // w}, {w}, {mode}>"
         ach = work[dom]["adds"] / dur_s / 1e12
         roof = dict(bound="mfma", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=ach / FP32_PEAK_TFLOPS, kernel=kname,
