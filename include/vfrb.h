@@ -244,10 +244,10 @@ typedef struct vfrb_search_desc {
     const float* dms;     /* host, ndm values                                */
     int32_t max_rows;     /* largest window (rows) ever passed               */
     int32_t storage;      /* VFRB_STORE_*                                    */
-    int32_t tile_config;  /* -1 = default (the channel-pair tile, or a
-                             per-channel tile for DM lists too coarse for
-                             it), else an index returned by
-                             vfrb_search_tile_count/vfrb_search_tile_name */
+    int32_t tile_config;  /* -1 = default (the 4-channel group tile; for DM
+                             lists too coarse for it the channel-pair tile,
+                             then a per-channel tile), else an index returned
+                             by vfrb_search_tile_count/vfrb_search_tile_name */
 } vfrb_search_desc;
 
 typedef struct vfrb_search_plan vfrb_search_plan;

@@ -1029,11 +1029,12 @@ constexpr TileCfg kTiles[] = {
     {4, 8, 16, 2, "G256_dm128_w16"},  {4, 4, 16, 2, "G256_dm64_w16"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
-// Default tile: the channel-pair tile, which needs the relative delay of
-// adjacent channels to vary by at most kPairMaxND rows across a DM block; for
-// DM lists too coarse for that, the per-channel kFallbackTile.
-constexpr int kDefaultTile = 12;  // P512_dm64_w16
-constexpr int kFallbackTile = 3;  // T512_dm64_w8
+// Default tile: the 4-channel group tile G256_dm64_w16.  Group and pair tiles
+// need the block's DMs to see each group (pair) at few relative delays; for
+// DM lists too coarse for them the default falls back along kTileFallback:
+// the pair tile, then the per-channel T512_dm64_w8, which fits any list.
+constexpr int kDefaultTile = 19;                // G256_dm64_w16
+constexpr int kTileFallback[] = {19, 12, 3};    // G256_dm64_w16, P512_dm64_w16, T512_dm64_w8
 
 template <typename TS, int I>
 const void* tile_kernel_ptr(bool batch) {
@@ -1439,8 +1440,13 @@ vfrb_status vfrb_search_plan_create(vfrb_search_plan** plan, const vfrb_search_d
         destroy(p);
         return st;
     }
-    st = build_tiles(p, d->tile_config >= 0 ? d->tile_config : kDefaultTile);
-    if (st == VFRB_ERR_UNSUPPORTED && d->tile_config < 0) st = build_tiles(p, kFallbackTile);
+    if (d->tile_config >= 0) {
+        st = build_tiles(p, d->tile_config);
+    } else {
+        st = VFRB_ERR_UNSUPPORTED;
+        for (int t : kTileFallback)
+            if (st == VFRB_ERR_UNSUPPORTED) st = build_tiles(p, t);
+    }
     if (st != VFRB_OK) {
         destroy(p);
         return st;
@@ -1797,8 +1803,11 @@ vfrb_status vfrb_batch_set(vfrb_batch* b, int32_t njobs, vfrb_search_plan* const
     b->outs.assign(outs, outs + njobs);
     b->ld_outs.assign(ld_outs, ld_outs + njobs);
     vfrb_status st = batch_build(b, vfrb::as_stream(stream));
-    if (st == VFRB_ERR_UNSUPPORTED && b->tile_default && b->tile == kDefaultTile) {
-        st = batch_configure(b, kFallbackTile);
+    // a default-tiled batch steps down the fallback list until every job fits
+    for (int fi = 1; fi < (int)(sizeof(kTileFallback) / sizeof(int)) && st == VFRB_ERR_UNSUPPORTED &&
+                     b->tile_default && b->tile == kTileFallback[fi - 1];
+         fi++) {
+        st = batch_configure(b, kTileFallback[fi]);
         if (st == VFRB_OK) st = batch_build(b, vfrb::as_stream(stream));
     }
     if (st != VFRB_OK) {
