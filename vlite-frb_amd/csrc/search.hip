@@ -813,20 +813,62 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
                 }
                 wo[m] = ok ? (r * ch.stride + 4 * g) * 4 : -1;
             }
-            float x[NI][kGroupCh][4];
+            if constexpr (ESZ == 2) {
+                // f16: after the phase shift, channel c's dwords hold elements
+                // (0,1) and (2,3).  v_perm pairs element e of two channels in
+                // one half2, and v_dot2c_f32_f16 with (1,1) adds both into an
+                // f32 sum: convert and add in one instruction per pair.
+                uint32_t raw[NI][kGroupCh][3];
 #pragma unroll
-            for (int m = 0; m < NI; m++)
+                for (int m = 0; m < NI; m++)
 #pragma unroll
-                for (int c = 0; c < kGroupCh; c++) load_elems<TS, 4>(rsrc, vo[m][c], ph[m][c], sc[m][c], x[m][c]);
+                    for (int c = 0; c < kGroupCh; c++)
 #pragma unroll
-            for (int m = 0; m < NI; m++) {
-                if (wo[m] < 0) continue;
-                float4 w;
-                w.x = (x[m][0][0] + x[m][1][0]) + (x[m][2][0] + x[m][3][0]);
-                w.y = (x[m][0][1] + x[m][1][1]) + (x[m][2][1] + x[m][3][1]);
-                w.z = (x[m][0][2] + x[m][1][2]) + (x[m][2][2] + x[m][3][2]);
-                w.w = (x[m][0][3] + x[m][1][3]) + (x[m][2][3] + x[m][3][3]);
-                *reinterpret_cast<float4*>(data + wo[m]) = w;
+                        for (int i = 0; i < 3; i++)
+                            raw[m][c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[m][c] + 4 * i, 0, 0);
+                constexpr h2_t kOnes = {(_Float16)1.0f, (_Float16)1.0f};
+#pragma unroll
+                for (int m = 0; m < NI; m++) {
+                    if (wo[m] < 0) continue;
+                    uint32_t al[kGroupCh][2];
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++)
+#pragma unroll
+                        for (int i = 0; i < 2; i++)
+                            al[c][i] = __builtin_amdgcn_alignbyte(raw[m][c][i + 1], raw[m][c][i], 2 * ph[m][c]);
+                    float v[4];
+#pragma unroll
+                    for (int i = 0; i < 2; i++) {
+#pragma unroll
+                        for (int hsel = 0; hsel < 2; hsel++) {
+                            const uint32_t sel = hsel ? 0x07060302u : 0x05040100u;
+                            const uint32_t q01 = __builtin_amdgcn_perm(al[1][i], al[0][i], sel);
+                            const uint32_t q23 = __builtin_amdgcn_perm(al[3][i], al[2][i], sel);
+                            const float s01 = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q01), kOnes,
+                                                                     0.0f, false);
+                            v[2 * i + hsel] = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q23), kOnes,
+                                                                     s01, false);
+                        }
+                    }
+                    *reinterpret_cast<float4*>(data + wo[m]) = make_float4(v[0], v[1], v[2], v[3]);
+                }
+            } else {
+                float x[NI][kGroupCh][4];
+#pragma unroll
+                for (int m = 0; m < NI; m++)
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++)
+                        load_elems<TS, 4>(rsrc, vo[m][c], ph[m][c], sc[m][c], x[m][c]);
+#pragma unroll
+                for (int m = 0; m < NI; m++) {
+                    if (wo[m] < 0) continue;
+                    float4 w;
+                    w.x = (x[m][0][0] + x[m][1][0]) + (x[m][2][0] + x[m][3][0]);
+                    w.y = (x[m][0][1] + x[m][1][1]) + (x[m][2][1] + x[m][3][1]);
+                    w.z = (x[m][0][2] + x[m][1][2]) + (x[m][2][2] + x[m][3][2]);
+                    w.w = (x[m][0][3] + x[m][1][3]) + (x[m][2][3] + x[m][3][3]);
+                    *reinterpret_cast<float4*>(data + wo[m]) = w;
+                }
             }
         };
         stage(0);
