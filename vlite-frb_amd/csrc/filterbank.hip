@@ -542,7 +542,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
 // Mixed-radix kernel for other even lengths (factors 2,3,4,5,8), one FFT per
 // workgroup of 256 threads, N <= 16384.
 // ---------------------------------------------------------------------------
-constexpr int GEN_BLOCK = 256;
+constexpr int GEN_BLOCK = 1024;
 constexpr int GEN_NMAX = 16384;
 constexpr int GEN_MAXPASS = 16;
 
