@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--payload-cache", default="",
                     help="file to reuse the synthesised payload between runs (written if absent)")
-    ap.add_argument("--sections", default="precision,batching,serving",
+    ap.add_argument("--sections", default="precision,batching,serving,stats",
                     help="extra measurements reported in the same line (empty = none)")
     ap.add_argument("--input", default="frames", choices=["frames", "payload"],
                     help="device-resident input: raw VDIF frame pairs (ingested on the GPU) or de-framed payload")
@@ -357,7 +357,8 @@ def main():
     del d0, d1, raw, ring, out, cdm, ct, csnr
     extra = {}
     for sec in [x for x in args.sections.split(",") if x]:
-        fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving}[sec]
+        fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving,
+              "stats": bench_stats}[sec]
         t = time.perf_counter()
         try:
             extra[sec] = fn(vfrb, args)
@@ -558,6 +559,96 @@ def bench_batching(vfrb, args):
         log(f"batching {sname}: tile {tile} sequential {seq:.2f} ms, batched {eager:.2f} ms, graph "
             f"{graph:.2f} ms ({seq / graph:.2f}x), pulses found {found}/16")
         del b, jobs
+    return res
+
+
+def _device_fb(vfrb, rows, nch, rng, block_rows=4096):
+    """f32 [rows][nch] noise filterbank in HBM (channel means 50-150, sigma
+    10): one random block uploaded and repeated by device copies."""
+    blk = (rng.standard_normal((block_rows, nch), dtype=np.float32) * 10.0
+           + rng.uniform(50.0, 150.0, nch).astype(np.float32))
+    d = vfrb.DeviceArray((rows, nch), np.float32)
+    b = vfrb.DeviceArray.from_host(blk)
+    row_b = nch * 4
+    for r0 in range(0, rows, block_rows):
+        nr = min(block_rows, rows - r0)
+        vfrb.call("vfrb_memcpy_d2d", d.offset(r0 * row_b), b.ptr, nr * row_b, None)
+    vfrb.call("vfrb_device_synchronize")
+    return d
+
+
+def bench_stats(vfrb, args):
+    """The STATS program's window work and the dedisperseSTATS variant at
+    scale.  (1) One 60 s window of 1 s statistic blocks: subtractbpCAL +
+    calcstats as one fused pass (vfrb_bandpass_chan_stats) against the two
+    launches, at the program's defaults (12500-point FFT, NAVE 4) and at
+    4096 channels (8192-point FFT, NAVE 16).  (2) dedisperseSTATS (mean and
+    spread over channels of every (DM, t)) of a 4096 ch x 2048 DM f32 window
+    of 20 s + max delay.  Inputs already in HBM."""
+    rng = np.random.default_rng(7)
+    s = vfrb.Stream()
+    ev0, ev1 = vfrb.Event(), vfrb.Event()
+
+    def timed(fn, reps=3):
+        fn()
+        ev0.record(s.h)
+        for _ in range(reps):
+            fn()
+        ev1.record(s.h)
+        ev1.sync()
+        return ev0.elapsed_ms(ev1) / reps
+
+    res = {}
+    for name, (n, nave, nb, fullband) in {"stats_default_12500pt": (12500, 4, 32, False),
+                                          "stats_4096ch": (8192, 16, 16, True)}.items():
+        p = vfrb.params_derive(n, nave, nb, 60.0, fullband=fullband)
+        rows, nch = p.maxnt, p.nchanout
+        nblk, ncol = 60, int(1.0 / p.tchan)
+        fb = _device_fb(vfrb, rows, nch, rng)
+        outs = tuple(vfrb.DeviceArray((nblk, nch), np.float32) for _ in range(3))
+        fused = timed(lambda: vfrb.bandpass_chan_stats(fb, rows, nch, ncol, nblk, stream=s.h, outs=outs))
+        ref = [o.download() for o in outs]
+        # two launches on a scratch copy (the in-place bandpass changes it)
+        tmp = vfrb.DeviceArray((rows, nch), np.float32)
+
+        def two():
+            vfrb.call("vfrb_memcpy_d2d", tmp.ptr, fb.ptr, fb.nbytes, s.h)
+            vfrb.bandpass_inplace(tmp, rows, nch, stream=s.h)
+            vfrb.call("vfrb_chan_stats", tmp.ptr, rows, nch, ncol, nblk, outs[0].ptr, outs[1].ptr,
+                      outs[2].ptr, s.h)
+        copy_ms = timed(lambda: vfrb.call("vfrb_memcpy_d2d", tmp.ptr, fb.ptr, fb.nbytes, s.h))
+        two_ms = timed(two) - copy_ms
+        same = all(np.array_equal(a, o.download(), equal_nan=True) for a, o in zip(ref, outs))
+        gb = rows * nch * 4 / 1e9  # compulsory: one read of the window
+        res[name] = {"fft": n, "nave": nave, "nchan": nch, "window_rows": rows, "stat_block_rows": ncol,
+                     "window_GB": gb, "fused_ms": fused, "two_launch_ms": two_ms, "speedup": two_ms / fused,
+                     "fused_GBps": gb / (fused * 1e-3), "fused_hbm_frac": gb / (fused * 1e-3) / HBM_PEAK_GBS,
+                     "stage_rtf": 60.0 / (fused * 1e-3), "identical_to_two_launch": bool(same)}
+        log(f"stats {name}: {nch} ch x {rows} rows, fused {fused:.2f} ms ({gb / (fused * 1e-3):.0f} GB/s), "
+            f"two launches {two_ms:.2f} ms, identical {same}")
+        del fb, tmp, outs
+
+    # dedisperseSTATS at 4096 ch x 2048 DMs
+    W = workload("c2")
+    p = vfrb.params_derive(W["twonchan"], W["nave"], 16, 1.0, fullband=True)
+    dms = np.linspace(0.0, W["dm_max"], W["ndm"]).astype(np.float32)
+    nch, tchan = p.nchanout, p.tchan
+    seg_rows = int(W["seg_s"] / tchan)
+    sp = vfrb.SearchPlan(nch, tchan, p.fch1, p.chanbw, dms, seg_rows + 20000, vfrb.STORE_F32)
+    rows = seg_rows + sp.max_delay
+    fb = _device_fb(vfrb, rows, nch, rng)
+    sp.load(fb, rows, bandpass=True, stream=s.h)
+    nt = rows - sp.max_delay
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    sig = vfrb.DeviceArray((len(dms), nt), np.float32)
+    ms = timed(lambda: sp.dedisperse_stats(rows, out, sig, nt, stream=s.h))
+    adds = 2.0 * len(dms) * nch * nt  # a sum and a sum of squares per (DM, channel, sample)
+    res["dedisperse_stats_4096ch_2048dm"] = {
+        "window_rows": rows, "nt": nt, "ms": ms, "segment_s": nt * tchan, "stage_rtf": nt * tchan / (ms * 1e-3),
+        "TFLOPs": adds / (ms * 1e-3) / 1e12, "fp32_frac": adds / (ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS}
+    log(f"dedisperse_stats 4096ch x 2048dm: {ms:.2f} ms per {nt * tchan:.1f} s window "
+        f"({nt * tchan / (ms * 1e-3):.0f} data-s/s)")
+    del sp, fb, out, sig
     return res
 
 

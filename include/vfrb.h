@@ -194,6 +194,16 @@ vfrb_status vfrb_chan_stats(const float* fb, int32_t nrows, int32_t nch,
                             int32_t ncolstatblk, int32_t nstatblks,
                             float* ave, float* sig, float* outlier, void* stream);
 
+/*
+ * subtractbpCAL followed by the per-channel statistics, as the STATS program
+ * runs them on every window (STATS:412-416), in one read of the raw
+ * filterbank: replaces vfrb_bandpass_inplace + vfrb_chan_stats with the same
+ * results bit for bit, and leaves fb unchanged.
+ */
+vfrb_status vfrb_bandpass_chan_stats(const float* fb, int32_t nrows, int32_t nch,
+                                     int32_t ncolstatblk, int32_t nstatblks,
+                                     float* ave, float* sig, float* outlier, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Device-side frame ingest (no reference counterpart: the reference copies   */
 /* payloads frame by frame on the host, PIPELINE:934-1171)                    */

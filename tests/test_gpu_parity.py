@@ -108,6 +108,68 @@ def test_chan_stats_vs_oracle(gpu, orc):
     assert np.array_equal(out, rout)
 
 
+@pytest.mark.parametrize("rows,nch,ncol,nblk", [
+    (2000, 300, 400, 5),     # stat blocks straddle bandpass blocks
+    (3000, 130, 977, 3),     # rows past the last stat block, ragged last bandpass block
+    (1000, 64, 128, 8),      # aligned blocks, short last one
+    (40, 70, 1, 40),         # one-row stat blocks: 0/0 variance, as the reference
+    (70000, 64, 33000, 2),   # long stat blocks (260 bandpass blocks each)
+])
+def test_bandpass_chan_stats_fused(gpu, orc, rows, nch, ncol, nblk):
+    """The one-pass STATS window (bandpass + channel statistics) equals the
+    two launches bit for bit, leaves the filterbank unchanged, and agrees
+    with the oracle's subtractbpCAL + calcstats."""
+    vfrb = gpu
+    rng = np.random.default_rng(rows + nch)
+    fb = rng.normal(100, 10, (rows, nch)).astype(np.float32)
+    fb[:, 3] = 1.0
+    fb[rows // 3, 5] += 500.0
+    fb[rows // 2:rows // 2 + 9, 7] += 90.0
+    d = vfrb.DeviceArray.from_host(fb)
+    fused = vfrb.bandpass_chan_stats(d, rows, nch, ncol, nblk)
+    assert np.array_equal(d.download(), fb)
+    vfrb.bandpass_inplace(d, rows, nch)
+    two = vfrb.chan_stats(d, rows, nch, ncol, nblk)
+    for a, b in zip(fused, two):
+        assert np.array_equal(a, b, equal_nan=True)
+    if rows <= 3000 or ncol > 30000:
+        rave, rsig, rout = orc.chan_stats(orc.bandpass_f32(fb), ncol, nblk)
+        ok = np.isfinite(rsig)
+        assert np.array_equal(ok, np.isfinite(fused[1]))
+        assert np.allclose(fused[0], rave, atol=1e-4)
+        assert np.allclose(fused[1][ok], rsig[ok], rtol=1e-4, atol=1e-4)
+        assert np.array_equal(fused[2][ok], rout[ok])
+
+
+def test_chan_stats_all_above_hi(gpu, orc):
+    """The outlier fraction is count/npts in integer division: 1 exactly when
+    every sample exceeds HI = ave + 5*sig.  A channel of equal samples whose
+    float mean rounds below them (977 x 0.1f or 1.1f in the reference's
+    sequential order; 0.3f rounds above) with sig 0 has fraction 1.  The
+    kernels decide it from the smallest sample, which must agree with the
+    count against the HI of their own ave and sig; a NaN sample is never
+    above HI."""
+    vfrb = gpu
+    rows, nch, ncol, nblk = 977, 64, 977, 1
+    fb = np.random.default_rng(5).normal(0, 1, (rows, nch)).astype(np.float32)
+    for c, v in ((9, 0.1), (10, 0.1), (11, 1.1), (12, 0.3)):
+        fb[:, c] = np.float32(v)
+    fb[100, 10] = np.nan
+    rave, rsig, rout = orc.chan_stats(fb, ncol, nblk)
+    assert rout[0, 9] == 1.0 and rout[0, 11] == 1.0 and rout[0, 12] == 0.0 and rout[0, 10] == 0.0
+    d = vfrb.DeviceArray.from_host(fb)
+    for ave, sig, out in (vfrb.chan_stats(d, rows, nch, ncol, nblk),):
+        for c in (9, 11, 12):
+            hi = np.float32(np.float64(ave[0, c]) + 5.0 * np.float64(sig[0, c]))
+            assert out[0, c] == float(np.all(fb[:, c] > hi)), (c, ave[0, c], sig[0, c])
+        assert out[0, 10] == 0.0
+        normal = np.ones(nch, bool)
+        normal[9:13] = False
+        assert np.array_equal(out[0, normal], rout[0, normal])
+        assert np.allclose(ave[0, normal], rave[0, normal], atol=1e-5)
+        assert np.allclose(sig[0, normal], rsig[0, normal], rtol=1e-4)
+
+
 # ------------------------------------------------------- dedispersion ----
 def _c1_shape():
     nch, tchan = 1024, 64 * 2048 / 128e6
@@ -276,6 +338,40 @@ def test_dedisperse_stats_variant(gpu, orc):
     rout, rsig = orc.dedisperse_stats_f32(fb, off, nt)
     assert np.allclose(out.download(), rout, atol=1e-4)
     assert np.allclose(sig.download(), rsig, rtol=1e-3)
+
+
+@pytest.mark.parametrize("mean,rtol", [(0.0, 2e-5), (40.0, 2e-3)])
+def test_dedisperse_stats_tiles(gpu, orc, mean, rtol):
+    """The tiled dedisperseSTATS over several DM blocks and time tiles, a
+    window longer than one tile row and a ragged last tile: ave within float
+    summation order of the oracle; sigma from one-pass sums, whose
+    cancellation grows with mean^2/variance (exact to ~1e-5 at mean 0, the
+    bandpassed window the programs feed it)."""
+    vfrb = gpu
+    nch, tchan = 1000, 4 * 12500 / 128e6
+    chanbw = 64.0 / 6250
+    fch1 = 320.0 + nch * chanbw - 0.5 * chanbw
+    dms = np.linspace(0.0, 90.0, 77).astype(np.float32)
+    off, maxd = orc.delay_table(dms, nch, fch1, chanbw, tchan)
+    rows = maxd + 1333
+    rng = np.random.default_rng(31)
+    fb = (rng.normal(0, 5, (rows, nch)) + mean).astype(np.float32)
+    fb[maxd // 2, 17] += 300.0  # one outlier seen by many (dm, t)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
+    nt = rows - plan.max_delay
+    d = vfrb.DeviceArray.from_host(fb)
+    plan.load(d, rows, bandpass=False)
+    ld = nt + 5
+    out, sig = vfrb.DeviceArray((len(dms), ld), np.float32), vfrb.DeviceArray((len(dms), ld), np.float32)
+    plan.dedisperse_stats(rows, out, sig, ld)
+    rout, rsig = orc.dedisperse_stats_f32(fb, off, nt)
+    go, gs = out.download()[:, :nt], sig.download()[:, :nt]
+    assert np.max(np.abs(go - rout)) <= 1e-5 * max(1.0, abs(mean)) + 1e-6
+    assert np.max(np.abs(gs - rsig) / rsig) < rtol, np.max(np.abs(gs - rsig) / rsig)
+    # the plain dedispersion of the same window is the mean series
+    o2 = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.dedisperse(rows, o2, nt)
+    assert np.max(np.abs(o2.download() - go)) <= 1e-5 * max(1.0, abs(mean)) + 1e-6
 
 
 # --------------------------------------------------------- statistics ----

@@ -147,8 +147,9 @@ int main(int argc, char** argv) {
         CLI_CHECK(vfrb_fb_execute(fbp, d0, d1, p.nbundle, dfb, 0, (int64_t)loopcnt * p.nbundle, nullptr), 1);
         loopcnt++;
         if (loopcnt == nloopwin) {
-            CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
-            CLI_CHECK(vfrb_chan_stats(dfb, p.maxnt, p.nchanout, ncolstatblk, nstatblks, dave, dsig, dout, nullptr), 1);
+            // STATS:414-416: subtractbpCAL then calcstats, as one pass
+            CLI_CHECK(vfrb_bandpass_chan_stats(dfb, p.maxnt, p.nchanout, ncolstatblk, nstatblks, dave, dsig, dout,
+                                               nullptr), 1);
             CLI_CHECK(vfrb_memcpy_d2h(have.data(), dave, blk_bytes, nullptr), 2);
             CLI_CHECK(vfrb_memcpy_d2h(hsig.data(), dsig, blk_bytes, nullptr), 2);
             CLI_CHECK(vfrb_memcpy_d2h(hout.data(), dout, blk_bytes, nullptr), 2);

@@ -279,7 +279,9 @@ struct WavesPerEu {
 // One tile: DM block b (W*DW DMs) x time tile tx (T = 64*SL samples) of one
 // window.  SL samples per lane, DW DMs per wave, W waves.  The tile ends
 // with a barrier, so the caller may reuse the LDS at once.
-template <typename TS, int SL, int DW, int W>
+// ST (f32 only): dedisperseSTATS as well, a second accumulator per sample
+// sums the squares and sig receives the spread over channels.
+template <typename TS, int SL, int DW, int W, bool ST = false>
 __device__ __forceinline__ void
 dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
             const uint32_t* __restrict__ roffs,  // per chunk: [count][W*DW] LDS byte offsets
@@ -287,7 +289,8 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
             const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
             const float* __restrict__ chan_scale,  // u8 only
             const int* __restrict__ f16_exp, const float* __restrict__ offsum,
-            float* __restrict__ out, int64_t ld_out, int b, int tx, unsigned char* lds_raw) {
+            float* __restrict__ out, int64_t ld_out, int b, int tx, unsigned char* lds_raw,
+            float* __restrict__ sig = nullptr) {
     // LDS: [kChunkMax][DMB] u32 byte offsets of each (channel, DM) read,
     // then the staged rows.  A row holds COPIES copies of the channel's
     // samples, copy p shifted by p elements, so that the dword a lane reads
@@ -314,11 +317,16 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     const int* dmin_b = dmin + (int64_t)b * nch;
     const uint32_t ldw = (uint32_t)(ld / EPD);  // dwords per window row
 
+    static_assert(!ST || ESZ == 4, "dedisperseSTATS reads an f32 window");
     float acc[DW][SL];
+    float acc2[ST ? DW : 1][ST ? SL : 1];
 #pragma unroll
     for (int d = 0; d < DW; d++)
 #pragma unroll
-        for (int j = 0; j < SL; j++) acc[d][j] = 0.0f;
+        for (int j = 0; j < SL; j++) {
+            acc[d][j] = 0.0f;
+            if constexpr (ST) acc2[d][j] = 0.0f;
+        }
 
     // first chunk's channel start rows
     if (cr.y > 0) {
@@ -431,7 +439,9 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
                 for (int q = 0; q < NDW; q++) {
                     const uint32_t w = p[64 * q];
                     if constexpr (ESZ == 4) {
-                        acc[d][q] += __uint_as_float(w);
+                        const float v = __uint_as_float(w);
+                        acc[d][q] += v;
+                        if constexpr (ST) acc2[d][q] = fmaf(v, v, acc2[d][q]);
                     } else if constexpr (ESZ == 2) {
                         // one v_dot2c_f32_f16 per sample: acc += h.lo*1 + h.hi*0
                         const h2_t h = *reinterpret_cast<const h2_t*>(&w);
@@ -469,6 +479,18 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
                     if constexpr (ESZ == 4) o[t] = a / fnch;
                     else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
                     else o[t] = (a + add) / fnch;
+                    if constexpr (ST) {
+                        // dedisperseSTATS (TIMESERIES:1462-1477): ave = S1/N in
+                        // float, sum = sum(x - ave), sum2 = sum((x - ave)^2),
+                        // sig = sqrt((sum2 - sum*sum) / (N(N-1))).  sum and sum2
+                        // follow from S1 and S2 = sum(x^2) exactly in real
+                        // arithmetic; they are formed in double.
+                        const double ave = (double)(a / fnch), s1 = a, n = fnch;
+                        const double sum = s1 - n * ave;
+                        const double sum2 = (double)acc2[d][EPD * q + e] - 2.0 * ave * s1 + n * ave * ave;
+                        const float num = fmaxf((float)(sum2 - sum * sum), 0.0f);
+                        sig[(int64_t)dm * ld_out + t] = sqrtf(num / (float)(nch * (nch - 1)));
+                    }
                 }
             }
         }
@@ -1024,27 +1046,20 @@ dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restri
     }
 }
 
-// dedisperseSTATS (TIMESERIES:1450-1481), one thread per (dm, t), f32 window.
-__global__ void dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, int nch, int nt,
-                                    int ndm, const int* __restrict__ off, float* __restrict__ out,
-                                    float* __restrict__ sig, int64_t ld_out) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)nt * ndm) return;
-    const int dm = (int)(idx / nt);
-    const int t = (int)(idx - (int64_t)dm * nt);
-    const int* o = off + (int64_t)dm * nch;
-    float sum = 0.0f;
-    for (int c = 0; c < nch; c++) sum += win[(int64_t)c * ld + t + o[c]];
-    const float ave = sum / (float)nch;
-    float s1 = 0.0f, s2 = 0.0f;
-    for (int c = 0; c < nch; c++) {
-        const float d = win[(int64_t)c * ld + t + o[c]] - ave;
-        s1 += d;
-        s2 += d * d;
-    }
-    const float sigma = (s2 - (s1 * s1)) / (float)(nch * (nch - 1));
-    out[(int64_t)dm * ld_out + t] = ave;
-    sig[(int64_t)dm * ld_out + t] = sqrtf(sigma);
+// dedisperseSTATS (TIMESERIES:1450-1481) on the per-channel tile of an f32
+// window: one workgroup per (DM block, time tile), out = mean over channels,
+// sig = their spread, both [ndm][ld_out].
+template <int SL, int DW, int W>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(WavesPerEu<W>::value, WavesPerEu<W>::value)))
+dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
+                    const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
+                    const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
+                    float* __restrict__ out, float* __restrict__ sig, int64_t ld_out, int nblk) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+    const int id = xcd_tile(blockIdx.x, gridDim.x);
+    const int b = id % nblk, tx = id / nblk;
+    dedisp_tile<float, SL, DW, W, true>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks, nullptr,
+                                        nullptr, nullptr, out, ld_out, b, tx, lds_raw, sig);
 }
 
 // ---------------------------------------------------------------------------
@@ -1077,6 +1092,10 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 // the pair tile, then the per-channel T512_dm64_w8, which fits any list.
 constexpr int kDefaultTile = 19;                // G256_dm64_w16
 constexpr int kTileFallback[] = {19, 12, 3};    // G256_dm64_w16, P512_dm64_w16, T512_dm64_w8
+// dedisperseSTATS tile: per-channel, two accumulators per sample, so half the
+// DMs per wave of T512_dm64_w8 within the same 128 VGPRs (fastest of
+// 256x64, 512x32 and 1024x16 samples x DMs at 4096 ch x 2048 DMs).
+constexpr TileCfg kStatsTile = {8, 4, 8, 0, "S512_dm32_w8"};
 
 template <typename TS, int I>
 const void* tile_kernel_ptr(bool batch) {
@@ -1103,6 +1122,19 @@ size_t storage_bytes(int storage) {
     return storage == VFRB_STORE_F32 ? 4 : storage == VFRB_STORE_F16 ? 2 : 1;
 }
 
+// Device tables of one tile configuration over a plan's DM list.
+struct TileSet {
+    int tile = -1;
+    uint32_t* d_roffs = nullptr;
+    int* d_dmin = nullptr;
+    Chunk* d_chunks = nullptr;
+    int2* d_blk_chunks = nullptr;
+    int nblk = 0;
+    size_t lds_bytes = 0;
+    // estimated lane-operations per tile of each DM block (batch ordering)
+    std::vector<int64_t> blk_cost;
+};
+
 }  // namespace
 
 struct vfrb_search_plan {
@@ -1112,7 +1144,6 @@ struct vfrb_search_plan {
     int32_t max_delay = 0;
     int nch = 0, ndm = 0, storage = 0;
     int64_t ld = 0;            // window row length (elements)
-    int tile = -1;
     // device
     int32_t* d_off = nullptr;
     void* d_win = nullptr;
@@ -1123,16 +1154,9 @@ struct vfrb_search_plan {
     float* d_u8_off = nullptr;
     float* d_u8_offsum = nullptr;
     float* d_tmp_fb = nullptr;  // time-major staging for vfrb_search_execute
-    // tile tables
-    uint32_t* d_roffs = nullptr;
-    int* d_dmin = nullptr;
-    Chunk* d_chunks = nullptr;
-    int2* d_blk_chunks = nullptr;
-    int nblk = 0;
-    size_t lds_bytes = 0;
+    TileSet ts;   // tables of the dedispersion tile in use
+    TileSet sts;  // tables of the dedisperseSTATS tile, built on first use
     int loaded_rows = 0;
-    // estimated lane-operations per tile of each DM block (batch ordering)
-    std::vector<int64_t> blk_cost;
 };
 
 struct vfrb_batch {
@@ -1162,15 +1186,12 @@ struct vfrb_batch {
 
 namespace {
 
-void free_tiles(vfrb_search_plan* p) {
-    (void)hipFree(p->d_roffs);
-    (void)hipFree(p->d_dmin);
-    (void)hipFree(p->d_chunks);
-    (void)hipFree(p->d_blk_chunks);
-    p->d_roffs = nullptr;
-    p->d_dmin = nullptr;
-    p->d_chunks = nullptr;
-    p->d_blk_chunks = nullptr;
+void free_tiles(TileSet& t) {
+    (void)hipFree(t.d_roffs);
+    (void)hipFree(t.d_dmin);
+    (void)hipFree(t.d_chunks);
+    (void)hipFree(t.d_blk_chunks);
+    t = TileSet{};
 }
 
 // Builds the per-DM-block tables for tile configuration i: minimum delay per
@@ -1178,8 +1199,7 @@ void free_tiles(vfrb_search_plan* p) {
 // (chunk channel, DM of the block) the LDS byte offset of the first dword the
 // kernel reads: channel row + copy (delay mod samples-per-dword) + aligned
 // position of the delay inside that copy.
-vfrb_status build_tiles(vfrb_search_plan* p, int i) {
-    const TileCfg& tc = kTiles[i];
+vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, TileSet& out) {
     const int T = 64 * tc.sl, DMB = tc.w * tc.dw;
     const int nch = p->nch, ndm = p->ndm;
     const int nblk = (ndm + DMB - 1) / DMB;
@@ -1377,32 +1397,35 @@ vfrb_status build_tiles(vfrb_search_plan* p, int i) {
     }
     if (tc.mode == 2) dmin.swap(recs);
     if (dmin.empty()) dmin.push_back(0);
-    free_tiles(p);
-    hipError_t e = hipMalloc(&p->d_roffs, roffs.size() * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMalloc(&p->d_dmin, dmin.size() * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&p->d_chunks, chunks.size() * sizeof(Chunk));
-    if (e == hipSuccess) e = hipMalloc(&p->d_blk_chunks, blk.size() * sizeof(int2));
-    if (e == hipSuccess) e = hipMemcpy(p->d_roffs, roffs.data(), roffs.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->d_dmin, dmin.data(), dmin.size() * sizeof(int), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(p->d_blk_chunks, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice);
+    free_tiles(out);
+    hipError_t e = hipMalloc(&out.d_roffs, roffs.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&out.d_dmin, dmin.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&out.d_chunks, chunks.size() * sizeof(Chunk));
+    if (e == hipSuccess) e = hipMalloc(&out.d_blk_chunks, blk.size() * sizeof(int2));
+    if (e == hipSuccess) e = hipMemcpy(out.d_roffs, roffs.data(), roffs.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(out.d_dmin, dmin.data(), dmin.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(out.d_chunks, chunks.data(), chunks.size() * sizeof(Chunk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(out.d_blk_chunks, blk.data(), blk.size() * sizeof(int2), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        free_tiles(p);
+        free_tiles(out);
         return vfrb::hip_fail(e, "build_tiles", __FILE__, __LINE__);
     }
-    p->nblk = nblk;
-    p->tile = i;
-    p->blk_cost = std::move(cost);
-    p->lds_bytes = lds_head_bytes(DMB) + max_data;
+    out.nblk = nblk;
+    out.tile = i;
+    out.blk_cost = std::move(cost);
+    out.lds_bytes = lds_head_bytes(DMB) + max_data;
     return VFRB_OK;
 }
 
+vfrb_status build_tiles(vfrb_search_plan* p, int i) { return build_tile_set(p, kTiles[i], i, p->ts); }
+
 const void* dedisp_kernel_for(const vfrb_search_plan* p) {
-    return tile_kernel_for(p->storage, p->tile, false);
+    return tile_kernel_for(p->storage, p->ts.tile, false);
 }
 
 void destroy(vfrb_search_plan* p) {
-    free_tiles(p);
+    free_tiles(p->ts);
+    free_tiles(p->sts);
     (void)hipFree(p->d_off);
     (void)hipFree(p->d_win);
     (void)hipFree(p->d_bp_mean);
@@ -1513,11 +1536,11 @@ vfrb_status vfrb_search_get_offsets(const vfrb_search_plan* p, int32_t* host) {
 
 vfrb_status vfrb_search_set_tile(vfrb_search_plan* p, int32_t i) {
     VFRB_REQUIRE(p && i >= 0 && i < kNumTiles, VFRB_ERR_INVALID, "tile %d", i);
-    if (p->tile == i) return VFRB_OK;
+    if (p->ts.tile == i) return VFRB_OK;
     return build_tiles(p, i);
 }
 
-int32_t vfrb_search_get_tile(const vfrb_search_plan* p) { return p ? p->tile : -1; }
+int32_t vfrb_search_get_tile(const vfrb_search_plan* p) { return p ? p->ts.tile : -1; }
 
 vfrb_status vfrb_search_load(vfrb_search_plan* p, const float* fb, int64_t ring_rows, int64_t row0,
                              int32_t nrows, int32_t apply_bp, void* stream) {
@@ -1570,22 +1593,22 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
     VFRB_REQUIRE(ld_out >= nt, VFRB_ERR_INVALID, "ld_out=%lld < %d", (long long)ld_out, nt);
     VFRB_REQUIRE(nrows <= p->loaded_rows, VFRB_ERR_INVALID,
                  "window holds %d rows, %d requested", p->loaded_rows, nrows);
-    const TileCfg& tc = kTiles[p->tile];
+    const TileCfg& tc = kTiles[p->ts.tile];
     const int T = 64 * tc.sl;
-    const int64_t nwg = vfrb::ceil_div(nt, T) * p->nblk;
+    const int64_t nwg = vfrb::ceil_div(nt, T) * p->ts.nblk;
     VFRB_REQUIRE(nwg < INT32_MAX, VFRB_ERR_UNSUPPORTED, "%lld dedispersion tiles", (long long)nwg);
     dim3 grid((unsigned)nwg), block(64 * tc.w);
-    int nblk = p->nblk;
+    int nblk = p->ts.nblk;
     const void* kern = dedisp_kernel_for(p);
-    VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes));
+    VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->ts.lds_bytes));
     const void* win = p->d_win;
     int64_t ld = p->ld;
     int nch = p->nch, ndm = p->ndm;
     void* args[] = {(void*)&win, (void*)&ld, (void*)&nch, (void*)&nrows, (void*)&nt,
-                    (void*)&ndm, (void*)&p->d_roffs, (void*)&p->d_dmin, (void*)&p->d_chunks,
-                    (void*)&p->d_blk_chunks, (void*)&p->d_u8_scale, (void*)&p->d_f16_exp,
+                    (void*)&ndm, (void*)&p->ts.d_roffs, (void*)&p->ts.d_dmin, (void*)&p->ts.d_chunks,
+                    (void*)&p->ts.d_blk_chunks, (void*)&p->d_u8_scale, (void*)&p->d_f16_exp,
                     (void*)&p->d_u8_offsum, (void*)&out, (void*)&ld_out, (void*)&nblk};
-    VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->lds_bytes, vfrb::as_stream(stream)));
+    VFRB_HIP(hipLaunchKernel(kern, grid, block, args, p->ts.lds_bytes, vfrb::as_stream(stream)));
     return VFRB_OK;
 }
 
@@ -1598,7 +1621,7 @@ vfrb_status vfrb_search_autotune(vfrb_search_plan* p, int32_t nrows, float* out,
     hipEvent_t e0, e1;
     VFRB_HIP(hipEventCreate(&e0));
     VFRB_HIP(hipEventCreate(&e1));
-    const int keep = p->tile;
+    const int keep = p->ts.tile;
     int best = -1;
     float best_ms = 0.0f;
     vfrb_status st = VFRB_OK;
@@ -1642,12 +1665,23 @@ vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* p, int32_t nrows, flo
     VFRB_REQUIRE(p && out && sig, VFRB_ERR_INVALID, "NULL argument");
     VFRB_REQUIRE(p->storage == VFRB_STORE_F32, VFRB_ERR_UNSUPPORTED, "needs an f32 plan");
     const int nt = nrows - p->max_delay;
-    VFRB_REQUIRE(nrows <= p->desc.max_rows && nt > 0 && ld_out >= nt, VFRB_ERR_INVALID,
+    VFRB_REQUIRE(nrows > 0 && nrows <= p->desc.max_rows && nt > 0 && ld_out >= nt, VFRB_ERR_INVALID,
                  "nrows=%d max_delay=%d ld_out=%lld", nrows, p->max_delay, (long long)ld_out);
-    const int64_t total = (int64_t)nt * p->ndm;
-    hipLaunchKernelGGL(dedisp_stats_kernel, dim3((unsigned)vfrb::ceil_div(total, 256)), dim3(256), 0,
-                       vfrb::as_stream(stream), (const float*)p->d_win, p->ld, p->nch, nt, p->ndm,
-                       (const int*)p->d_off, out, sig, ld_out);
+    VFRB_REQUIRE(nrows <= p->loaded_rows, VFRB_ERR_INVALID,
+                 "window holds %d rows, %d requested", p->loaded_rows, nrows);
+    if (p->sts.tile < 0) {
+        const vfrb_status st = build_tile_set(p, kStatsTile, 0, p->sts);
+        if (st != VFRB_OK) return st;
+    }
+    constexpr TileCfg tc = kStatsTile;
+    const int64_t nwg = vfrb::ceil_div(nt, 64 * tc.sl) * p->sts.nblk;
+    VFRB_REQUIRE(nwg < INT32_MAX, VFRB_ERR_UNSUPPORTED, "%lld dedispersion tiles", (long long)nwg);
+    const void* kern = reinterpret_cast<const void*>(&dedisp_stats_kernel<tc.sl, tc.dw, tc.w>);
+    VFRB_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->sts.lds_bytes));
+    hipLaunchKernelGGL((dedisp_stats_kernel<tc.sl, tc.dw, tc.w>), dim3((unsigned)nwg), dim3(64 * tc.w),
+                       p->sts.lds_bytes, vfrb::as_stream(stream), (const float*)p->d_win, p->ld, p->nch, nrows,
+                       nt, p->ndm, p->sts.d_roffs, p->sts.d_dmin, p->sts.d_chunks, p->sts.d_blk_chunks, out, sig,
+                       ld_out, p->sts.nblk);
     VFRB_LAUNCH_CHECK("dedisp_stats_kernel");
     return VFRB_OK;
 }
@@ -1704,7 +1738,7 @@ vfrb_status batch_build(vfrb_batch* b, hipStream_t s) {
     int64_t total = 0;
     for (int j = 0; j < njobs; j++) {
         vfrb_search_plan* p = b->plans[j];
-        if (p->tile != b->tile) {
+        if (p->ts.tile != b->tile) {
             const vfrb_status st = build_tiles(p, b->tile);
             if (st != VFRB_OK) return st;
         }
@@ -1717,17 +1751,17 @@ vfrb_status batch_build(vfrb_batch* b, hipStream_t s) {
         jb.nrows = nrows;
         jb.nt = nt;
         jb.ndm = p->ndm;
-        jb.roffs = p->d_roffs;
-        jb.dmin = p->d_dmin;
-        jb.chunks = p->d_chunks;
-        jb.blk_chunks = p->d_blk_chunks;
+        jb.roffs = p->ts.d_roffs;
+        jb.dmin = p->ts.d_dmin;
+        jb.chunks = p->ts.d_chunks;
+        jb.blk_chunks = p->ts.d_blk_chunks;
         jb.chan_scale = p->d_u8_scale;
         jb.f16_exp = p->d_f16_exp;
         jb.offsum = p->d_u8_offsum;
         jb.out = b->outs[j];
         jb.ld_out = b->ld_outs[j];
-        for (int k = 0; k < p->nblk; k++) blks.push_back(Blk{p->blk_cost[k], j, k});
-        total += (int64_t)p->nblk * vfrb::ceil_div(nt, T);
+        for (int k = 0; k < p->ts.nblk; k++) blks.push_back(Blk{p->ts.blk_cost[k], j, k});
+        total += (int64_t)p->ts.nblk * vfrb::ceil_div(nt, T);
     }
     VFRB_REQUIRE(total <= b->max_tiles, VFRB_ERR_CAPACITY,
                  "%lld work items exceed the batch capacity %d", (long long)total, b->max_tiles);

@@ -36,47 +36,136 @@ bandpass_inplace_kernel(float* __restrict__ fb, int nrows, int nch) {
         if (i < n_mine) fb[(int64_t)(rbase + mine0 + i) * nch + c] = x[i] - m;
 }
 
-// STATS calcstats (STATS:910-946).  Block = 64 channels x 4 row quarters.
+// Calls f(r) for the rows r of [lo, hi) that row quarter q owns: the 32-row
+// chunks k = r / 32 with k % 4 == q, i.e. exactly the rows quarter q holds of
+// every 128-row bandpass block, in increasing order.
+// A whole chunk is unrolled, so that its 32 loads are in flight together.
+template <class F>
+__device__ __forceinline__ void for_owned_rows(int lo, int hi, int q, F f) {
+    const int k0 = lo >> 5;
+    for (int k = k0 + ((q - k0) & 3); (k << 5) < hi; k += 4) {
+        const int kb = k << 5;
+        if (kb >= lo && kb + 32 <= hi) {
+#pragma unroll
+            for (int i = 0; i < 32; i++) f(kb + i);
+        } else {
+            for (int r = max(kb, lo); r < min(kb + 32, hi); r++) f(r);
+        }
+    }
+}
+
+// One thread's share of a channel's stat block for calcstats (STATS:910-946),
+// in one pass: the plain float sum s (the mean, as the reference forms it),
+// sums a and b of x - K and (x - K)^2 about a shift K (the block's first
+// sample, so that the spread needs no second pass and has no cancellation
+// beyond the data's own), and the smallest sample (a NaN counts as -inf).
+struct ChanAcc {
+    float s = 0.0f, a = 0.0f, b = 0.0f, mn = INFINITY;
+    __device__ __forceinline__ void add(float x, float k) {
+        s += x;
+        const float t = x - k;
+        a += t;
+        b = fmaf(t, t, b);
+        mn = fminf(mn, x != x ? -INFINITY : x);
+    }
+};
+
+__device__ __forceinline__ float bp_reduce_min(float v, float* red, int lane, int q) {
+    __syncthreads();
+    red[q * BP_CH + lane] = v;
+    __syncthreads();
+    return fminf(fminf(red[lane], red[BP_CH + lane]), fminf(red[2 * BP_CH + lane], red[3 * BP_CH + lane]));
+}
+
+// calcstats of one channel and stat block of npts rows from the four
+// quarters' ChanAcc: ave = s/npts; with d = ave - K, sum = sum(x - ave) =
+// a - npts*d and sum2 = sum((x - ave)^2) = b - 2*d*a + npts*d^2 (exact in
+// real arithmetic, formed in double); sig = (sum2 - sum*sum/npts)/(npts-1)
+// (the variance); the outlier fraction is the reference's count/npts in
+// integer division, which is 1 exactly when every sample exceeds
+// HI = ave + 5*sig, i.e. when the smallest does (a NaN sample never does).
+__device__ __forceinline__ void chan_stats_finish(const ChanAcc& acc, float k, int npts, bool cvalid, int lane,
+                                                  int q, float* red, int64_t o, float* __restrict__ ave,
+                                                  float* __restrict__ sig, float* __restrict__ outlier) {
+    const float m = vfrb::bp_reduce(acc.s, red, lane, q) / (float)npts;
+    const float a = vfrb::bp_reduce(acc.a, red, lane, q);
+    const float b = vfrb::bp_reduce(acc.b, red, lane, q);
+    const float mn = bp_reduce_min(acc.mn, red, lane, q);
+    const double n = npts, d = (double)m - (double)k;
+    const double sum = (double)a - n * d;
+    const double sum2 = (double)b - 2.0 * d * (double)a + n * d * d;
+    const float v = (float)((sum2 - (sum * sum) / n) / (n - 1.0));
+    const float hi = (float)((double)m + (5.0 * (double)v));
+    if (q == 0 && cvalid) {
+        ave[o] = m;
+        sig[o] = v;
+        outlier[o] = mn > hi ? 1.0f : 0.0f;  // (count / npts), STATS:943
+    }
+}
+
+// STATS calcstats (STATS:910-946) on a bandpassed filterbank.  Block = 64
+// channels x 4 row quarters, one stat block; one read of the data.
 __global__ void __launch_bounds__(256)
 chan_stats_kernel(const float* __restrict__ fb, int nrows, int nch, int ncol, float* __restrict__ ave,
                   float* __restrict__ sig, float* __restrict__ outlier) {
+    __shared__ float red[BP_Q * BP_CH];
+    const int lane = threadIdx.x & 63;
+    const int q = threadIdx.x >> 6;
+    const int c = blockIdx.x * BP_CH + lane;
+    const int colmin = ncol * blockIdx.y;
+    const int colmax = min(colmin + ncol, nrows);
+    const bool cvalid = c < nch;
+    auto at = [&](int r) { return fb[(int64_t)r * nch + c]; };
+    const float k = cvalid ? at(colmin) : 0.0f;
+    ChanAcc acc;
+    if (cvalid) for_owned_rows(colmin, colmax, q, [&](int r) { acc.add(at(r), k); });
+    chan_stats_finish(acc, k, colmax - colmin, cvalid, lane, q, red, (int64_t)blockIdx.y * nch + c, ave, sig,
+                      outlier);
+}
+
+// subtractbpCAL (PIPELINE:1427-1474) and STATS calcstats (STATS:910-946) in
+// one read of the raw filterbank, which is left unchanged: the workgroup of
+// (64 channels, stat block) computes the clipped mean of every 128-row
+// bandpass block the stat block touches, with the layout and arithmetic of
+// bandpass_inplace_kernel (so the same bandpassed floats), and feeds each
+// bandpassed sample of the stat block to the statistics as it goes.  Each
+// quarter sees its rows in the order chan_stats_kernel's quarter does, so
+// the results equal bandpass_inplace_kernel followed by chan_stats_kernel
+// bit for bit.
+__global__ void __launch_bounds__(256)
+bandpass_chan_stats_kernel(const float* __restrict__ fb, int nrows, int nch, int ncol,
+                           float* __restrict__ ave, float* __restrict__ sig, float* __restrict__ outlier) {
     __shared__ float red[BP_Q * BP_CH];
     __shared__ int ired[BP_Q * BP_CH];
     const int lane = threadIdx.x & 63;
     const int q = threadIdx.x >> 6;
     const int c = blockIdx.x * BP_CH + lane;
-    const int blk = blockIdx.y;
-    const int colmin = ncol * blk;
+    const int colmin = ncol * blockIdx.y;
     const int colmax = min(colmin + ncol, nrows);
-    const int npts = colmax - colmin;
     const bool cvalid = c < nch;
-    auto at = [&](int r) { return fb[(int64_t)r * nch + c]; };
-    float s = 0.0f;
-    if (cvalid)
-        for (int r = colmin + q; r < colmax; r += BP_Q) s += at(r);
-    const float m = vfrb::bp_reduce(s, red, lane, q) / (float)npts;
-    float s1 = 0.0f, s2 = 0.0f;
-    if (cvalid)
-        for (int r = colmin + q; r < colmax; r += BP_Q) {
-            const float d = at(r) - m;
-            s2 += d * d;
-            s1 += d;
+    const int b0 = colmin / BP_ROWS, b1 = (colmax - 1) / BP_ROWS;
+    const int mine0 = q * BP_PER;
+    float k = 0.0f;
+    ChanAcc acc;
+    for (int b = b0; b <= b1; b++) {
+        const int rbase = b * BP_ROWS;
+        const int npts = min(BP_ROWS, nrows - rbase);
+        const int n_mine = cvalid ? max(0, min(BP_PER, npts - mine0)) : 0;
+        float x[BP_PER];
+#pragma unroll
+        for (int i = 0; i < BP_PER; i++)
+            x[i] = i < n_mine ? fb[(int64_t)(rbase + mine0 + i) * nch + c] : 0.0f;
+        const float m = vfrb::bp_clipped_mean(x, n_mine, npts, red, ired, lane, q);
+        // the shift: the first bandpassed sample of the stat block
+        if (b == b0 && cvalid) k = fb[(int64_t)colmin * nch + c] - m;
+#pragma unroll
+        for (int i = 0; i < BP_PER; i++) {
+            const int r = rbase + mine0 + i;
+            if (i < n_mine && r >= colmin && r < colmax) acc.add(x[i] - m, k);
         }
-    const float sum = vfrb::bp_reduce(s1, red, lane, q);
-    const float sum2 = vfrb::bp_reduce(s2, red, lane, q);
-    const float v = (sum2 - (sum * sum) / (float)npts) / (float)(npts - 1);
-    const float hi = (float)((double)m + (5.0 * (double)v));
-    int cnt = 0;
-    if (cvalid)
-        for (int r = colmin + q; r < colmax; r += BP_Q)
-            if (at(r) > hi) cnt++;
-    const int ctot = vfrb::bp_reduce_int(cnt, ired, lane, q);
-    if (q == 0 && cvalid) {
-        const int64_t o = (int64_t)blk * nch + c;
-        ave[o] = m;
-        sig[o] = v;
-        outlier[o] = (float)(ctot / npts);  // integer division, as STATS:943
     }
+    chan_stats_finish(acc, k, colmax - colmin, cvalid, lane, q, red, (int64_t)blockIdx.y * nch + c, ave, sig,
+                      outlier);
 }
 
 // Neumaier-compensated float sum.
@@ -252,6 +341,19 @@ vfrb_status vfrb_chan_stats(const float* fb, int32_t nrows, int32_t nch, int32_t
     hipLaunchKernelGGL(chan_stats_kernel, grid, dim3(256), 0, vfrb::as_stream(stream), fb, nrows, nch,
                        ncol, ave, sig, outlier);
     VFRB_LAUNCH_CHECK("chan_stats_kernel");
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_bandpass_chan_stats(const float* fb, int32_t nrows, int32_t nch, int32_t ncol, int32_t nblk,
+                                     float* ave, float* sig, float* outlier, void* stream) {
+    VFRB_REQUIRE(fb && ave && sig && outlier && nch > 0 && ncol > 0 && nblk > 0, VFRB_ERR_INVALID,
+                 "bad chan_stats arguments");
+    VFRB_REQUIRE((int64_t)ncol * (nblk - 1) < nrows, VFRB_ERR_INVALID,
+                 "stat block %d starts beyond the %d rows", nblk - 1, nrows);
+    dim3 grid((unsigned)vfrb::ceil_div(nch, BP_CH), (unsigned)nblk);
+    hipLaunchKernelGGL(bandpass_chan_stats_kernel, grid, dim3(256), 0, vfrb::as_stream(stream), fb, nrows,
+                       nch, ncol, ave, sig, outlier);
+    VFRB_LAUNCH_CHECK("bandpass_chan_stats_kernel");
     return VFRB_OK;
 }
 

@@ -135,6 +135,7 @@ _SIGS = {
     "vfrb_ingest_gather": (i32, [vp, vp, i64, i32, vp, vp]),
     "vfrb_bandpass_inplace": (i32, [vp, i32, i32, vp]),
     "vfrb_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
+    "vfrb_bandpass_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "vfrb_search_plan_create": (i32, [P(vp), P(SearchDesc)]),
     "vfrb_search_plan_destroy": (i32, [vp]),
     "vfrb_search_max_delay": (i32, [vp]),
@@ -741,6 +742,18 @@ def bandpass_inplace(fb: DeviceArray, nrows, nch, stream=None):
 def chan_stats(fb: DeviceArray, nrows, nch, ncol, nblk, stream=None):
     ave, sig, out = (DeviceArray((nblk, nch), np.float32) for _ in range(3))
     call("vfrb_chan_stats", fb.ptr, nrows, nch, ncol, nblk, ave.ptr, sig.ptr, out.ptr, _s(stream))
+    return ave.download(), sig.download(), out.download()
+
+
+def bandpass_chan_stats(fb: DeviceArray, nrows, nch, ncol, nblk, stream=None, outs=None):
+    """bandpass_inplace + chan_stats in one pass; fb is left unchanged.
+    outs: three device arrays [nblk][nch] to fill (then nothing is
+    downloaded and they are returned), else new ones are downloaded."""
+    arrs = outs or tuple(DeviceArray((nblk, nch), np.float32) for _ in range(3))
+    ave, sig, out = arrs
+    call("vfrb_bandpass_chan_stats", fb.ptr, nrows, nch, ncol, nblk, ave.ptr, sig.ptr, out.ptr, _s(stream))
+    if outs:
+        return outs
     return ave.download(), sig.download(), out.download()
 
 
