@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--payload-cache", default="",
                     help="file to reuse the synthesised payload between runs (written if absent)")
-    ap.add_argument("--sections", default="precision,batching,serving,stats",
+    ap.add_argument("--sections", default="c1,precision,batching,serving,stats",
                     help="extra measurements reported in the same line (empty = none)")
     ap.add_argument("--input", default="frames", choices=["frames", "payload"],
                     help="device-resident input: raw VDIF frame pairs (ingested on the GPU) or de-framed payload")
@@ -370,7 +370,7 @@ def main():
     extra = {}
     for sec in [x for x in args.sections.split(",") if x]:
         fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving,
-              "stats": bench_stats}[sec]
+              "stats": bench_stats, "c1": bench_c1}[sec]
         t = time.perf_counter()
         try:
             extra[sec] = fn(vfrb, args)
@@ -548,6 +548,81 @@ def bench_precision(vfrb, args):
                       "missing": len(ref_set - got), "extra": len(got - ref_set)}
         log(f"precision {sname}: max rel SNR err {rel.max():.2e} (median {np.median(rel):.2e}), "
             f"candidates {cnt} identical={got == ref_set}")
+    return res
+
+
+def bench_c1(vfrb, args):
+    """The 1024 ch x 256 DM workload (2048-point FFT, NAVE 64: 1.024 ms rows;
+    DMs 0-25.5 step 0.1; one pulse at DM 12.3, SNR 20, seed 0), 1 s segments
+    of 976 new rows plus the 308-row overlap.  GPU: the chain on one stream
+    (filterbank of the payload, bandpass + pack, dedispersion, stats,
+    candidates), f32 and f16 storage, payload in HBM, timed over 20
+    segments.  CPU: the oracle's FP32 stage mirror on the same payload bytes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    n, nave, p = c1_shape(vfrb)
+    nch, tchan = p.nchanout, p.tchan
+    rows = int(1.0 / tchan)
+    dms = (np.arange(256) * 0.1).astype(np.float32)
+    off, max_delay, _ = vfrb.delay_table(dms, nch, p.fch1, p.chanbw, tchan)
+    win = rows + max_delay
+    amp = vfrb.amp_for_snr(20.0, nave, nch)
+    spec = vfrb.synth_spec(n, seed=0, pulses=[(12.3, 0.35, 0.002, amp)])
+    p0 = vfrb.synth_payload(spec, 0, 0, rows * nave)
+    p1 = vfrb.synth_payload(spec, 1, 0, rows * nave)
+    d0, d1 = vfrb.DeviceArray.from_host(p0), vfrb.DeviceArray.from_host(p1)
+    ring = vfrb.DeviceArray((win, nch), np.float32)
+    ring.zero()
+    fbp = vfrb.FilterbankPlan(n, nave, p.nskip, nch)
+    out = vfrb.DeviceArray((len(dms), rows), np.float32)
+    maxc = 1 << 16
+    cdm, ct = vfrb.DeviceArray(maxc, np.int32), vfrb.DeviceArray(maxc, np.int32)
+    csnr, cnt = vfrb.DeviceArray(maxc, np.float32), vfrb.DeviceArray(1, np.int32)
+    ave, sig, peak = (vfrb.DeviceArray(len(dms), np.float32) for _ in range(3))
+    frb, pt = vfrb.DeviceArray(len(dms), np.int32), vfrb.DeviceArray(len(dms), np.int32)
+    s = vfrb.Stream()
+    ev0, ev1 = vfrb.Event(), vfrb.Event()
+    res = {"workload": "1.0 s segments, 1024 ch x 256 DM (0-25.5), pulse DM 12.3 SNR 20, seed 0",
+           "segment_rows": rows, "max_delay_rows": max_delay}
+    for sname, acc in (("f32", vfrb.ACC_F64), ("f16", vfrb.ACC_F32C)):
+        sp = vfrb.SearchPlan(nch, tchan, p.fch1, p.chanbw, dms, win, vfrb.STORAGE_NAMES[sname])
+
+        def step(i):
+            fbp.execute(d0, d1, rows, ring, ring_rows=win, row0=(max_delay + i * rows) % win, stream=s.h)
+            sp.load(ring, win, ring_rows=win, row0=(i * rows) % win, bandpass=True, stream=s.h)
+            sp.dedisperse(win, out, rows, stream=s.h)
+            vfrb.call("vfrb_dm_stats", out.ptr, rows, len(dms), rows, acc, frb.ptr, ave.ptr, sig.ptr, peak.ptr,
+                      pt.ptr, s.h)
+            vfrb.call("vfrb_candidates", out.ptr, rows, len(dms), rows, ave.ptr, sig.ptr, 7.0, cdm.ptr, ct.ptr,
+                      csnr.ptr, maxc, cnt.ptr, s.h)
+        for i in range(3):
+            step(i)
+        nsteps = 20
+        ev0.record(s.h)
+        for i in range(nsteps):
+            step(3 + i)
+        ev1.record(s.h)
+        ev1.sync()
+        ms = ev0.elapsed_ms(ev1) / nsteps
+        pk = peak.download()
+        res[sname] = {"ms_per_segment": ms, "rtf": rows * tchan / (ms * 1e-3), "tile": vfrb.tile_names()[sp.tile],
+                      "best_dm": float(dms[int(np.argmax(pk))]), "best_snr": float(np.max(pk))}
+        log(f"c1 {sname}: {ms:.3f} ms per 1 s segment, RTF {rows * tchan / (ms * 1e-3):.0f}, best DM "
+            f"{dms[int(np.argmax(pk))]:.1f} SNR {np.max(pk):.1f}")
+        del sp
+    if not args.no_cpu_baseline:
+        t0 = time.perf_counter()
+        fb = orc.filterbank_f32(p0, p1, rows, n, nave, p.nskip, nch)
+        window = np.concatenate([fb[rows - max_delay:], fb])
+        fbcm = np.ascontiguousarray(orc.bandpass_f32(window).T)
+        dd = orc.dedisperse_cm_f32(fbcm, off, rows)
+        orc.dm_stats(dd)
+        total = time.perf_counter() - t0
+        res["cpu"] = {"rtf": rows * tchan / total, "seconds": total, "threads": orc.num_threads(),
+                      "kind": "port", "sample": "one 1 s segment: FP32 stage-mirror filterbank, bandpass over the "
+                                                "1284-row window, transpose, dedispersion, per-DM stats"}
+        log(f"c1 cpu ({orc.num_threads()} threads): {total:.2f} s per 1 s segment, RTF {rows * tchan / total:.2f}")
     return res
 
 
