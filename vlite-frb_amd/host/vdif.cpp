@@ -190,7 +190,11 @@ struct vfrb_seq {
         if (fnum != expected[pol] || (sec_check && fsec != expected_sec[pol])) {
             int gap = fnum - expected[pol];
             if (sec_check) {
-                gap += (fsec - expected_sec[pol]) * VFRB_FRAMESPERSEC;
+                // The reference forms this in int (TIMESERIES:901), which a
+                // damaged seconds field overflows; its result there is the
+                // 32-bit two's-complement wrap, formed here without overflow.
+                const int64_t g = (int64_t)gap + ((int64_t)fsec - expected_sec[pol]) * VFRB_FRAMESPERSEC;
+                gap = (int)(int32_t)(uint32_t)(uint64_t)g;
                 if (gap < 0) gap = 0;  // frames going backwards: no zero run
             } else if (gap < 0) {
                 gap += VFRB_FRAMESPERSEC;
