@@ -61,7 +61,10 @@ __global__ void __launch_bounds__(256) ingest_gap_kernel(const uint8_t* __restri
     if (exp >= 0 && (n0 != exp || (sec_check && s0 != exps))) {
         g = n0 - exp;
         if (sec_check) {
-            g += (s0 - exps) * kFps;
+            // 32-bit two's-complement wrap of the reference's int arithmetic
+            // (TIMESERIES:901) on a damaged seconds field, without overflow
+            const int64_t g64 = (int64_t)g + ((int64_t)s0 - exps) * kFps;
+            g = (int)(int32_t)(uint32_t)(uint64_t)g64;
             if (g < 0) g = 0;  // frames going backwards: no zero run
         } else if (g < 0) {
             g += kFps;
