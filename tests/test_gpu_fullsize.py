@@ -12,7 +12,10 @@ properties that hold exactly whatever the size:
   and through the persistent batch kernel;
 - DM 0 is the channel mean of the window row;
 - the delay table is the oracle's bit for bit, max_delay 12 090;
-- the per-DM statistics of a full output match the oracle's arithmetic."""
+- the per-DM statistics of a full output match the oracle's arithmetic;
+- the filterbank of a whole 20 s segment read as raw VDIF frame pairs (5.15 GB,
+  past 2^32 bytes, with dropped pairs the device ingest zero-fills) equals the
+  filterbank of the same payload de-framed on the host, bit for bit."""
 import numpy as np
 import pytest
 
@@ -122,3 +125,45 @@ def test_dm_stats_full_size(gpu, orc, full):
     assert np.max(np.abs(st.sigma - sig) / sig) < 2e-7
     assert np.array_equal(st.frb, frb) and np.array_equal(st.peak_t, pt)
     assert st.peak_t[1234] == 9876 and np.allclose(st.peak, peak, rtol=1e-6)
+
+
+def test_filterbank_from_frames_full_segment(gpu, full):
+    vfrb = gpu
+    p, dms, off, maxd, seg, rows = full
+    n, nave = 8192, 16
+    need = seg * nave * n                   # payload bytes per pol for the segment
+    npairs = -(-need // 5000)
+    drop = np.zeros(npairs, bool)
+    drop[[1000, 1001, 1002, 333333, npairs - 7]] = True
+    rng = np.random.default_rng(79)
+    pay = rng.integers(0, 256, (2, npairs, 5000), dtype=np.uint8)
+    pay[:, drop] = 0                        # what the ingest inserts for a missing pair
+    kept = np.flatnonzero(~drop)
+    frames = np.empty((kept.size, 2, 5032), np.uint8)
+    words = np.zeros((kept.size, 8), np.uint32)
+    words[:, 0] = 1000 + kept // 25600
+    words[:, 1] = (kept % 25600) | (30 << 24)
+    words[:, 2] = 629
+    for th in (0, 1):
+        words[:, 3] = 0x5856 | (7 << 26) | (th << 16)
+        frames[:, th, :32] = words.view(np.uint8).reshape(kept.size, 32)
+        frames[:, th, 32:] = pay[th, kept]
+    print(f"[case] framed filterbank, {kept.size} pairs ({frames.nbytes / 1e9:.2f} GB), {seg} rows", flush=True)
+    raw = vfrb.DeviceArray.from_host(frames.reshape(-1))
+    del frames
+    ing = vfrb.Ingest(kept.size)
+    table = vfrb.DeviceArray(npairs + 64, np.int32)
+    nslots = ing.frames(raw, kept.size, table)
+    assert nslots == npairs
+    fbp = vfrb.FilterbankPlan(n, nave, p.nskip, p.nchanout)
+    a = vfrb.DeviceArray((seg, NCH), np.float32)
+    fbp.execute_frames(raw, table, nslots, 0, seg, a)
+    got = a.download()
+    del raw, a
+    d0 = vfrb.DeviceArray.from_host(pay[0].reshape(-1))
+    d1 = vfrb.DeviceArray.from_host(pay[1].reshape(-1))
+    del pay
+    b = vfrb.DeviceArray((seg, NCH), np.float32)
+    fbp.execute(d0, d1, seg, b)
+    assert np.array_equal(got, b.download())
+    assert np.all(np.isfinite(got)) and got.min() >= 0.0
