@@ -70,8 +70,8 @@ def main():
                     help="wall seconds of each replayed arrival trace")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(sharding is described in DESIGN.md); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: libvfrb has no multi-GPU path (streams or DM ranges "
+              "would shard across processes, one GPU each); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import vfrb

@@ -101,7 +101,7 @@ int main(int argc, char** argv) {
     std::fprintf(flog, "DADA buffer at start:\n Connecting to DADA buffer with key: %x ...\n", 0xdada);
     if (a.input.empty()) {
         std::fprintf(flog, " Could not connect to DADA buffer!\n");
-        std::fprintf(stderr, "PSRDADA ring input is not part of this build; give the VDIF stream with "
+        std::fprintf(stderr, "PSRDADA ring input is not supported by this program; give the VDIF stream with "
                              "--input-file FILE (or - for stdin)\n");
         return -1;
     }

@@ -5,7 +5,7 @@
 // -nave, -nb, -k hex, -o), NCHANOUT = 3840, 30 s windows with the
 // max-delay overlap, seconds-aware gap filling, DM%.1f_<UTC>.dat and
 // _SIGMA.dat appended per window, .inf headers and a new set of files at each
-// UTC hour.  The PSRDADA shared-memory ring is not part of this build: the
+// UTC hour.  The PSRDADA shared-memory ring is not supported: the
 // VDIF stream is read with --input-file FILE ("-" = stdin).  Extensions:
 //   --input-file FILE   VDIF source (file or pipe)
 //   --skip-pairs N      frame pairs discarded first (default 256000, initDADA)
@@ -173,7 +173,7 @@ int main(int argc, char** argv) {
     std::fprintf(flog, "DADA buffer at start:\n Connecting to DADA buffer with key: %x ...\n", a.key);
     if (a.input.empty()) {
         std::fprintf(flog, " Could not connect to DADA buffer!\n");
-        std::fprintf(stderr, "PSRDADA ring input is not part of this build; give the VDIF stream with "
+        std::fprintf(stderr, "PSRDADA ring input is not supported by this program; give the VDIF stream with "
                              "--input-file FILE (or - for stdin)\n");
         return -1;
     }
