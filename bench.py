@@ -203,7 +203,7 @@ def main():
         if frames_mode:
             # every step replays the same frames: restart the parse state
             # (a live stream carries it from one batch to the next)
-            fr = frames or raw
+            fr = raw if frames is None else frames
             ing.reset()
             nsl = ing.frames(fr, npairs, table, stream=st)
             if mark is not None:
