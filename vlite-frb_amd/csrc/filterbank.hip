@@ -553,10 +553,9 @@ struct GenPlan {
 
 template <int R>
 __device__ __forceinline__ void gen_pass(float* __restrict__ re, float* __restrict__ im, int N, int Ns,
-                                         const float2* __restrict__ tw) {
+                                         const float2* __restrict__ tw, int tid) {
     constexpr int MAXQ = (GEN_NMAX / R + GEN_BLOCK - 1) / GEN_BLOCK;
     const int nb = N / R;
-    const int tid = threadIdx.x;
     cf v[MAXQ][R];
 #pragma unroll
     for (int q = 0; q < MAXQ; q++) {
@@ -595,11 +594,104 @@ __device__ __forceinline__ void gen_pass(float* __restrict__ re, float* __restri
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(GEN_BLOCK)
-fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
-                  const float2* __restrict__ tw, GenPlan plan, int N, float* __restrict__ fb,
-                  int64_t nrows, int nave, int nskip, int nchanout, int64_t ring_rows, int64_t row0,
-                  float scale) {
+// The same pass with every size a constant: a fixed length N, the product NS
+// of the radices before it, radix R.  Butterfly counts, trip counts, the
+// division by NS and the twiddle stride fold at compile time, so a pass holds
+// only its samples in registers.  The operations are gen_pass's, in the same
+// order (the compiler may fuse a different set of them into FMAs).
+template <int N, int NS, int R>
+__device__ __forceinline__ void fixed_pass(float* __restrict__ re, float* __restrict__ im,
+                                           const float2* __restrict__ tw, int tid) {
+    constexpr int NB = N / R;
+    constexpr int Q = (NB + GEN_BLOCK - 1) / GEN_BLOCK;
+    constexpr int TWSTEP = N / (NS * R);
+    cf v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int j = tid + GEN_BLOCK * q;
+        if (NB % GEN_BLOCK == 0 || j < NB) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = lds_pad(j + r * NB);
+                v[q][r] = cf{re[idx], im[idx]};
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int j = tid + GEN_BLOCK * q;
+        if (NB % GEN_BLOCK == 0 || j < NB) {
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const float2 w = tw[r * k * TWSTEP];
+                v[q][r] = cmul(v[q][r], cf{w.x, w.y});
+            }
+            dft<R>(v[q]);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = lds_pad(base + r * NS);
+                re[idx] = v[q][r].x;
+                im[idx] = v[q][r].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// The first pass of a fixed length reads its radix-R inputs straight from
+// the frame's bytes (z = (x0-128) + i(x1-128)), with no staging pass through
+// LDS.  With NS = 1 every twiddle is 1, so none is applied.
+template <int N, int R>
+__device__ __forceinline__ void fixed_first_pass(const uint8_t* __restrict__ x0, const uint8_t* __restrict__ x1,
+                                                 float* __restrict__ re, float* __restrict__ im, int tid) {
+    constexpr int NB = N / R;
+    constexpr int Q = (NB + GEN_BLOCK - 1) / GEN_BLOCK;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int j = tid + GEN_BLOCK * q;
+        if (NB % GEN_BLOCK == 0 || j < NB) {
+            cf v[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) v[r] = cf{(float)x0[j + r * NB] - 128.0f, (float)x1[j + r * NB] - 128.0f};
+            dft<R>(v);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = lds_pad(j * R + r);
+                re[idx] = v[r].x;
+                im[idx] = v[r].y;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NS, int R, int... Rest>
+__device__ __forceinline__ void fixed_passes(float* __restrict__ re, float* __restrict__ im,
+                                             const float2* __restrict__ tw, int tid) {
+    fixed_pass<N, NS, R>(re, im, tw, tid);
+    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, NS * R, Rest...>(re, im, tw, tid);
+}
+
+template <int N, int R0, int... Rest>
+__device__ __forceinline__ void fixed_rest(const uint8_t* __restrict__ x0, const uint8_t* __restrict__ x1,
+                                           float* __restrict__ re, float* __restrict__ im,
+                                           const float2* __restrict__ tw, int tid) {
+    fixed_first_pass<N, R0>(x0, x1, re, im, tid);
+    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, R0, Rest...>(re, im, tw, tid);
+}
+
+// One output row per workgroup: for each of the row's nave frames, both pols
+// as z = (x0-128) + i(x1-128), the complex FFT by `passes` (in LDS), then
+// |Z[k]|^2 + |Z[N-k]|^2 = |X0[k]|^2 + |X1[k]|^2 (times 2) accumulated per
+// selected channel.
+template <class Passes>
+__device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
+                                           int N, float* __restrict__ fb, int64_t nrows, int nave, int nskip,
+                                           int nchanout, int64_t ring_rows, int64_t row0, float scale,
+                                           Passes passes) {
     extern __shared__ float lds[];
     const int words = N + N / 32;
     float* re = lds;
@@ -613,23 +705,12 @@ fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ 
     for (int i = 0; i < CPT; i++) acc[i] = 0.0f;
     for (int frame = 0; frame < nave; frame++) {
         const size_t off = ((size_t)row * nave + frame) * N;
-        for (int e = tid; e < N; e += GEN_BLOCK) {
-            const int idx = lds_pad(e);
-            re[idx] = (float)pol0[off + e] - 128.0f;
-            im[idx] = (float)pol1[off + e] - 128.0f;
-        }
-        __syncthreads();
-        int Ns = 1;
-        for (int p = 0; p < plan.npass; p++) {
-            switch (plan.radix[p]) {
-                case 2: gen_pass<2>(re, im, N, Ns, tw); break;
-                case 3: gen_pass<3>(re, im, N, Ns, tw); break;
-                case 4: gen_pass<4>(re, im, N, Ns, tw); break;
-                case 5: gen_pass<5>(re, im, N, Ns, tw); break;
-                default: gen_pass<8>(re, im, N, Ns, tw); break;
-            }
-            Ns *= plan.radix[p];
-        }
+        // The passes' per-thread LDS addresses depend only on the thread
+        // index; an opaque copy of it per frame keeps the compiler from
+        // hoisting all of them out of the frame loop into (spilled) registers.
+        int t = tid;
+        asm volatile("" : "+v"(t));
+        passes(pol0 + off, pol1 + off, re, im, t);
 #pragma unroll
         for (int i = 0; i < CPT; i++) {
             const int c = tid + GEN_BLOCK * i;
@@ -649,6 +730,49 @@ fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ 
         const int c = tid + GEN_BLOCK * i;
         if (c < nchanout) dst[c] = acc[i] * scale;
     }
+}
+
+// Any length the plan factorises: the radix sequence is read at run time.
+__global__ void __launch_bounds__(GEN_BLOCK)
+fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
+                  const float2* __restrict__ tw, GenPlan plan, int N, float* __restrict__ fb,
+                  int64_t nrows, int nave, int nskip, int nchanout, int64_t ring_rows, int64_t row0,
+                  float scale) {
+    mixed_rows(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+               [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
+                   for (int e = t; e < N; e += GEN_BLOCK) {
+                       const int idx = lds_pad(e);
+                       re[idx] = (float)x0[e] - 128.0f;
+                       im[idx] = (float)x1[e] - 128.0f;
+                   }
+                   __syncthreads();
+                   int Ns = 1;
+                   for (int p = 0; p < plan.npass; p++) {
+                       switch (plan.radix[p]) {
+                           case 2: gen_pass<2>(re, im, N, Ns, tw, t); break;
+                           case 3: gen_pass<3>(re, im, N, Ns, tw, t); break;
+                           case 4: gen_pass<4>(re, im, N, Ns, tw, t); break;
+                           case 5: gen_pass<5>(re, im, N, Ns, tw, t); break;
+                           default: gen_pass<8>(re, im, N, Ns, tw, t); break;
+                       }
+                       Ns *= plan.radix[p];
+                   }
+               });
+}
+
+// A length with its radix sequence fixed at compile time (the order
+// factorise() gives); same arguments as fb_generic_kernel, whose plan and N
+// it ignores in favour of its own.
+template <int N, int... R>
+__global__ void __launch_bounds__(GEN_BLOCK)
+fb_fixed_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
+                const float2* __restrict__ tw, GenPlan, int, float* __restrict__ fb, int64_t nrows, int nave,
+                int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale) {
+    static_assert((R * ...) == N, "radices must multiply to N");
+    mixed_rows(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+               [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
+                   fixed_rest<N, R...>(x0, x1, re, im, tw, t);
+               });
 }
 
 }  // namespace
@@ -724,7 +848,10 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
                             N, GEN_NMAX);
             return VFRB_ERR_UNSUPPORTED;
         }
-        p->kernel = reinterpret_cast<const void*>(&fb_generic_kernel);
+        // 12500 points (NCHAN 6250, the reference programs' default) has a
+        // kernel of its own; other lengths read their radices at run time
+        p->kernel = N == 12500 ? reinterpret_cast<const void*>(&fb_fixed_kernel<12500, 5, 5, 5, 5, 5, 4>)
+                               : reinterpret_cast<const void*>(&fb_generic_kernel);
         p->lds_bytes = 2 * sizeof(float) * (size_t)(N + N / 32);
         p->block = GEN_BLOCK;
     }
@@ -779,10 +906,11 @@ vfrb_status vfrb_fb_execute(vfrb_fb_plan* p, const uint8_t* pol0, const uint8_t*
                         (void*)&row0, (void*)&scale, (void*)&no_table, (void*)&no_byte0};
         VFRB_HIP(hipLaunchKernel(p->kernel, grid, block, args, p->lds_bytes, s));
     } else {
-        hipLaunchKernelGGL(fb_generic_kernel, grid, block, p->lds_bytes, s, pol0, pol1,
-                           (const float2*)p->d_tw, p->gen, p->N, fb, nrows, p->nave, p->nskip,
-                           p->nchanout, ring_rows, row0, scale);
-        VFRB_LAUNCH_CHECK("fb_generic_kernel");
+        const float2* tw = p->d_tw;
+        void* args[] = {(void*)&pol0, (void*)&pol1, (void*)&tw, (void*)&p->gen, (void*)&p->N, (void*)&fb,
+                        (void*)&nrows, (void*)&p->nave, (void*)&p->nskip, (void*)&p->nchanout,
+                        (void*)&ring_rows, (void*)&row0, (void*)&scale};
+        VFRB_HIP(hipLaunchKernel(p->kernel, grid, block, args, p->lds_bytes, s));
     }
     return VFRB_OK;
 }
