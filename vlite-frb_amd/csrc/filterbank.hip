@@ -543,6 +543,10 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
 // workgroup of 256 threads, N <= 16384.
 // ---------------------------------------------------------------------------
 constexpr int GEN_BLOCK = 1024;
+// Threads of a fixed-length workgroup.  At 12500 points, 640 and 768 threads
+// (fewer idle lanes in the 2500- and 3125-butterfly passes) ran 0.97 and
+// 0.95 ms per data-second against 0.92 for 1024.
+constexpr int FIXED_BLOCK = 1024;
 constexpr int GEN_NMAX = 16384;
 constexpr int GEN_MAXPASS = 16;
 
@@ -599,17 +603,17 @@ __device__ __forceinline__ void gen_pass(float* __restrict__ re, float* __restri
 // division by NS and the twiddle stride fold at compile time, so a pass holds
 // only its samples in registers.  The operations are gen_pass's, in the same
 // order (the compiler may fuse a different set of them into FMAs).
-template <int N, int NS, int R>
+template <int N, int BLOCK, int NS, int R>
 __device__ __forceinline__ void fixed_pass(float* __restrict__ re, float* __restrict__ im,
                                            const float2* __restrict__ tw, int tid) {
     constexpr int NB = N / R;
-    constexpr int Q = (NB + GEN_BLOCK - 1) / GEN_BLOCK;
+    constexpr int Q = (NB + BLOCK - 1) / BLOCK;
     constexpr int TWSTEP = N / (NS * R);
     cf v[Q][R];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int j = tid + GEN_BLOCK * q;
-        if (NB % GEN_BLOCK == 0 || j < NB) {
+        const int j = tid + BLOCK * q;
+        if (NB % BLOCK == 0 || j < NB) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int idx = lds_pad(j + r * NB);
@@ -620,8 +624,8 @@ __device__ __forceinline__ void fixed_pass(float* __restrict__ re, float* __rest
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int j = tid + GEN_BLOCK * q;
-        if (NB % GEN_BLOCK == 0 || j < NB) {
+        const int j = tid + BLOCK * q;
+        if (NB % BLOCK == 0 || j < NB) {
             const int k = j % NS;
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -644,15 +648,15 @@ __device__ __forceinline__ void fixed_pass(float* __restrict__ re, float* __rest
 // The first pass of a fixed length reads its radix-R inputs straight from
 // the frame's bytes (z = (x0-128) + i(x1-128)), with no staging pass through
 // LDS.  With NS = 1 every twiddle is 1, so none is applied.
-template <int N, int R>
+template <int N, int BLOCK, int R>
 __device__ __forceinline__ void fixed_first_pass(const uint8_t* __restrict__ x0, const uint8_t* __restrict__ x1,
                                                  float* __restrict__ re, float* __restrict__ im, int tid) {
     constexpr int NB = N / R;
-    constexpr int Q = (NB + GEN_BLOCK - 1) / GEN_BLOCK;
+    constexpr int Q = (NB + BLOCK - 1) / BLOCK;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const int j = tid + GEN_BLOCK * q;
-        if (NB % GEN_BLOCK == 0 || j < NB) {
+        const int j = tid + BLOCK * q;
+        if (NB % BLOCK == 0 || j < NB) {
             cf v[R];
 #pragma unroll
             for (int r = 0; r < R; r++) v[r] = cf{(float)x0[j + r * NB] - 128.0f, (float)x1[j + r * NB] - 128.0f};
@@ -668,26 +672,26 @@ __device__ __forceinline__ void fixed_first_pass(const uint8_t* __restrict__ x0,
     __syncthreads();
 }
 
-template <int N, int NS, int R, int... Rest>
+template <int N, int BLOCK, int NS, int R, int... Rest>
 __device__ __forceinline__ void fixed_passes(float* __restrict__ re, float* __restrict__ im,
                                              const float2* __restrict__ tw, int tid) {
-    fixed_pass<N, NS, R>(re, im, tw, tid);
-    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, NS * R, Rest...>(re, im, tw, tid);
+    fixed_pass<N, BLOCK, NS, R>(re, im, tw, tid);
+    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, BLOCK, NS * R, Rest...>(re, im, tw, tid);
 }
 
-template <int N, int R0, int... Rest>
+template <int N, int BLOCK, int R0, int... Rest>
 __device__ __forceinline__ void fixed_rest(const uint8_t* __restrict__ x0, const uint8_t* __restrict__ x1,
                                            float* __restrict__ re, float* __restrict__ im,
                                            const float2* __restrict__ tw, int tid) {
-    fixed_first_pass<N, R0>(x0, x1, re, im, tid);
-    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, R0, Rest...>(re, im, tw, tid);
+    fixed_first_pass<N, BLOCK, R0>(x0, x1, re, im, tid);
+    if constexpr (sizeof...(Rest) > 0) fixed_passes<N, BLOCK, R0, Rest...>(re, im, tw, tid);
 }
 
 // One output row per workgroup: for each of the row's nave frames, both pols
 // as z = (x0-128) + i(x1-128), the complex FFT by `passes` (in LDS), then
 // |Z[k]|^2 + |Z[N-k]|^2 = |X0[k]|^2 + |X1[k]|^2 (times 2) accumulated per
 // selected channel.
-template <class Passes>
+template <int BLOCK, class Passes>
 __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                                            int N, float* __restrict__ fb, int64_t nrows, int nave, int nskip,
                                            int nchanout, int64_t ring_rows, int64_t row0, float scale,
@@ -699,7 +703,7 @@ __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, con
     const int tid = threadIdx.x;
     const int64_t row = blockIdx.x;
     if (row >= nrows) return;
-    constexpr int CPT = (GEN_NMAX / 2 + GEN_BLOCK - 1) / GEN_BLOCK;
+    constexpr int CPT = (GEN_NMAX / 2 + BLOCK - 1) / BLOCK;
     float acc[CPT];
 #pragma unroll
     for (int i = 0; i < CPT; i++) acc[i] = 0.0f;
@@ -713,7 +717,7 @@ __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, con
         passes(pol0 + off, pol1 + off, re, im, t);
 #pragma unroll
         for (int i = 0; i < CPT; i++) {
-            const int c = tid + GEN_BLOCK * i;
+            const int c = tid + BLOCK * i;
             if (c < nchanout) {
                 const int k = nskip + c;
                 const int ia = lds_pad(k), ib = lds_pad(k == 0 ? 0 : N - k);
@@ -727,7 +731,7 @@ __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, con
     float* dst = fb + orow * (int64_t)nchanout;
 #pragma unroll
     for (int i = 0; i < CPT; i++) {
-        const int c = tid + GEN_BLOCK * i;
+        const int c = tid + BLOCK * i;
         if (c < nchanout) dst[c] = acc[i] * scale;
     }
 }
@@ -738,7 +742,7 @@ fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ 
                   const float2* __restrict__ tw, GenPlan plan, int N, float* __restrict__ fb,
                   int64_t nrows, int nave, int nskip, int nchanout, int64_t ring_rows, int64_t row0,
                   float scale) {
-    mixed_rows(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+    mixed_rows<GEN_BLOCK>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
                [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
                    for (int e = t; e < N; e += GEN_BLOCK) {
                        const int idx = lds_pad(e);
@@ -763,16 +767,16 @@ fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ 
 // A length with its radix sequence fixed at compile time (the order
 // factorise() gives); same arguments as fb_generic_kernel, whose plan and N
 // it ignores in favour of its own.
-template <int N, int... R>
-__global__ void __launch_bounds__(GEN_BLOCK)
+template <int N, int BLOCK, int... R>
+__global__ void __launch_bounds__(BLOCK)
 fb_fixed_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                 const float2* __restrict__ tw, GenPlan, int, float* __restrict__ fb, int64_t nrows, int nave,
                 int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale) {
     static_assert((R * ...) == N, "radices must multiply to N");
-    mixed_rows(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
-               [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
-                   fixed_rest<N, R...>(x0, x1, re, im, tw, t);
-               });
+    mixed_rows<BLOCK>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+                      [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
+                          fixed_rest<N, BLOCK, R...>(x0, x1, re, im, tw, t);
+                      });
 }
 
 }  // namespace
@@ -850,10 +854,11 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
         }
         // 12500 points (NCHAN 6250, the reference programs' default) has a
         // kernel of its own; other lengths read their radices at run time
-        p->kernel = N == 12500 ? reinterpret_cast<const void*>(&fb_fixed_kernel<12500, 5, 5, 5, 5, 5, 4>)
-                               : reinterpret_cast<const void*>(&fb_generic_kernel);
+        p->kernel = N == 12500
+                        ? reinterpret_cast<const void*>(&fb_fixed_kernel<12500, FIXED_BLOCK, 5, 5, 5, 5, 5, 4>)
+                        : reinterpret_cast<const void*>(&fb_generic_kernel);
         p->lds_bytes = 2 * sizeof(float) * (size_t)(N + N / 32);
-        p->block = GEN_BLOCK;
+        p->block = N == 12500 ? FIXED_BLOCK : GEN_BLOCK;
     }
     // Twiddles exp(-2*pi*i*m/N), m < N, computed in FP64 and rounded once.
     std::vector<float2> tw(N);
