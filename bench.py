@@ -60,7 +60,7 @@ def main():
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--payload-cache", default="",
                     help="file to reuse the synthesised payload between runs (written if absent)")
-    ap.add_argument("--sections", default="c1,precision,batching,serving,stats",
+    ap.add_argument("--sections", default="c1,precision,batching,serving,stats,cli",
                     help="extra measurements reported in the same line (empty = none)")
     ap.add_argument("--input", default="frames", choices=["frames", "payload"],
                     help="device-resident input: raw VDIF frame pairs (ingested on the GPU) or de-framed payload")
@@ -370,7 +370,7 @@ def main():
     extra = {}
     for sec in [x for x in args.sections.split(",") if x]:
         fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving,
-              "stats": bench_stats, "c1": bench_c1}[sec]
+              "stats": bench_stats, "c1": bench_c1, "cli": bench_cli}[sec]
         t = time.perf_counter()
         try:
             extra[sec] = fn(vfrb, args)
@@ -623,6 +623,56 @@ def bench_c1(vfrb, args):
                       "kind": "port", "sample": "one 1 s segment: FP32 stage-mirror filterbank, bandpass over the "
                                                 "1284-row window, transpose, dedispersion, per-DM stats"}
         log(f"c1 cpu ({orc.num_threads()} threads): {total:.2f} s per 1 s segment, RTF {rows * tchan / total:.2f}")
+    return res
+
+
+def bench_cli(vfrb, args):
+    """The drop-in program end to end: vdifFILE2dedisp_PIPELINE -d (12500-point
+    FFT, NAVE 4, 32-row bundles, 6250 channels) on an 8 s VDIF file of
+    uniform-noise frame pairs in a temporary directory, 101 DMs 0-100, 4 s windows.  Wall
+    time of the process, start-up and plan creation included; input from the
+    file through the host frame sequencer and host-to-device copies, as the
+    reference reads it."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "vlite-frb_amd", "_build", "vdifFILE2dedisp_PIPELINE")
+    if not os.path.exists(exe):
+        return {"error": "vdifFILE2dedisp_PIPELINE not built"}
+    seconds = 8.0
+    npairs = int(seconds * 25600)
+    rng = np.random.default_rng(11)
+    with tempfile.TemporaryDirectory(dir="/tmp") as d:
+        path = os.path.join(d, "in.vdif")
+        with open(path, "wb") as f:
+            step = 25600
+            for k0 in range(0, npairs, step):
+                k = np.arange(k0, min(npairs, k0 + step))
+                words = np.zeros((k.size, 8), np.uint32)
+                words[:, 0] = 1000 + k // 25600
+                words[:, 1] = (k % 25600) | (30 << 24)
+                words[:, 2] = 629
+                blk = np.empty((k.size, 2, 5032), np.uint8)
+                for th in (0, 1):
+                    words[:, 3] = 0x5856 | (7 << 26) | (th << 16)
+                    blk[:, th, :32] = words.view(np.uint8).reshape(k.size, 32)
+                    blk[:, th, 32:] = rng.integers(96, 160, (k.size, 5000), dtype=np.uint8)
+                blk.tofile(f)
+        with open(os.path.join(d, "DM101.txt"), "w") as f:
+            for i in range(101):
+                f.write(f"{float(i):f}\n")
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "-d", "-i", path, "-o", "cli", "--window", "4"], cwd=d, capture_output=True,
+                           text=True, timeout=600)
+        wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return {"error": f"exit {r.returncode}: {r.stderr[-500:]}"}
+        log_lines = open(os.path.join(d, "cli.log")).read().splitlines()
+    data_s = npairs * 5000 / 128e6
+    res = {"program": "vdifFILE2dedisp_PIPELINE -d -i in.vdif --window 4", "data_s": data_s, "wall_s": wall,
+           "rtf": data_s / wall, "file_GB": npairs * 2 * 5032 / 1e9,
+           "log_tail": [x for x in log_lines if "Processed" in x or "Total run time" in x]}
+    log(f"cli: {data_s:.2f} data-s in {wall:.2f} s wall -> RTF {data_s / wall:.1f} (file {res['file_GB']:.2f} GB)")
     return res
 
 
