@@ -427,7 +427,7 @@ def test_default_tile_falls_back_for_coarse_dm_lists(gpu, orc):
     print("[case] default tile fallback", flush=True)
     plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows)
     names = vfrb.tile_names()
-    assert names[plan.tile] in ("G256_dm64_w16", "T512_dm64_w8")
+    assert names[plan.tile] in ("G256_dm64_w16", "T512_dm64_w16")
     keep = plan.tile
     with pytest.raises(vfrb.VfrbError):
         plan.set_tile(names.index("P512_dm64_w16"))
@@ -442,7 +442,7 @@ def test_default_tile_falls_back_for_coarse_dm_lists(gpu, orc):
     out.zero()
     batch = vfrb.Batch(vfrb.STORE_F32)
     batch.set([(plan, rows, out, nt)])
-    assert names[batch.tile] in ("G256_dm64_w16", "T512_dm64_w8")
+    assert names[batch.tile] in ("G256_dm64_w16", "T512_dm64_w16")
     batch.dedisperse()
     vfrb.sync()
     assert np.max(np.abs(out.download() - ref)) < 1e-4 * 22

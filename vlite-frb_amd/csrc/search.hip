@@ -1089,9 +1089,11 @@ constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 // Default tile: the 4-channel group tile G256_dm64_w16.  Group and pair tiles
 // need the block's DMs to see each group (pair) at few relative delays; for
 // DM lists too coarse for them the default falls back along kTileFallback:
-// the pair tile, then the per-channel T512_dm64_w8, which fits any list.
+// the pair tile, then the per-channel T512_dm64_w16, which fits any list
+// that a per-channel tile of 512 samples x 64 DMs fits (T512_dm64_w8 stages
+// the same rows, 14.5 against 11.0 ms in f32 at 4096 ch x 2048 DMs).
 constexpr int kDefaultTile = 19;                // G256_dm64_w16
-constexpr int kTileFallback[] = {19, 12, 3};    // G256_dm64_w16, P512_dm64_w16, T512_dm64_w8
+constexpr int kTileFallback[] = {19, 12, 8};    // G256_dm64_w16, P512_dm64_w16, T512_dm64_w16
 // dedisperseSTATS tile: per-channel, two accumulators per sample, so half the
 // DMs per wave of T512_dm64_w8 within the same 128 VGPRs (fastest of
 // 256x64, 512x32 and 1024x16 samples x DMs at 4096 ch x 2048 DMs).
