@@ -769,7 +769,9 @@ def bench_stats(vfrb, args):
     scale.  (1) One 60 s window of 1 s statistic blocks: subtractbpCAL +
     calcstats as one fused pass (vfrb_bandpass_chan_stats) against the two
     launches, at the program's defaults (12500-point FFT, NAVE 4) and at
-    4096 channels (8192-point FFT, NAVE 16).  (2) dedisperseSTATS (mean and
+    4096 channels (8192-point FFT, NAVE 16); with the window's filterbank
+    (timed on 2 s of payload, scaled to the window), the program's
+    device-resident rate.  (2) dedisperseSTATS (mean and
     spread over channels of every (DM, t)) of a 4096 ch x 2048 DM f32 window
     of 20 s + max delay.  Inputs already in HBM."""
     rng = np.random.default_rng(7)
@@ -807,13 +809,25 @@ def bench_stats(vfrb, args):
         two_ms = timed(two) - copy_ms
         same = all(np.array_equal(a, o.download(), equal_nan=True) for a, o in zip(ref, outs))
         gb = rows * nch * 4 / 1e9  # compulsory: one read of the window
+        del fb, tmp
+        # the program's filterbank, timed on 2 s of payload resident in HBM
+        # and scaled to the window (rows are independent)
+        frows = int(2.0 / p.tchan)
+        pay = [vfrb.DeviceArray.from_host(rng.integers(96, 160, frows * nave * n, dtype=np.uint8)) for _ in (0, 1)]
+        fbp = vfrb.FilterbankPlan(n, nave, p.nskip, nch)
+        fbo = vfrb.DeviceArray((frows, nch), np.float32)
+        fb_ms = timed(lambda: fbp.execute(pay[0], pay[1], frows, fbo, stream=s.h)) * rows / frows
+        del pay, fbo, fbp
         res[name] = {"fft": n, "nave": nave, "nchan": nch, "window_rows": rows, "stat_block_rows": ncol,
                      "window_GB": gb, "fused_ms": fused, "two_launch_ms": two_ms, "speedup": two_ms / fused,
                      "fused_GBps": gb / (fused * 1e-3), "fused_hbm_frac": gb / (fused * 1e-3) / HBM_PEAK_GBS,
-                     "stage_rtf": 60.0 / (fused * 1e-3), "identical_to_two_launch": bool(same)}
+                     "stage_rtf": 60.0 / (fused * 1e-3), "identical_to_two_launch": bool(same),
+                     "filterbank_ms_per_window": fb_ms,
+                     "program_rtf": rows * p.tchan / ((fb_ms + fused) * 1e-3)}
         log(f"stats {name}: {nch} ch x {rows} rows, fused {fused:.2f} ms ({gb / (fused * 1e-3):.0f} GB/s), "
-            f"two launches {two_ms:.2f} ms, identical {same}")
-        del fb, tmp, outs
+            f"two launches {two_ms:.2f} ms, identical {same}; filterbank {fb_ms:.1f} ms per window -> "
+            f"STATS program {rows * p.tchan / ((fb_ms + fused) * 1e-3):.0f} data-s/s (device-resident)")
+        del outs
 
     # dedisperseSTATS at 4096 ch x 2048 DMs
     W = workload("c2")
