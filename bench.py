@@ -172,11 +172,14 @@ def main():
         table = vfrb.DeviceArray(npairs + 4096, np.int32)
     log(f"{'loaded' if reuse else 'synthesised'} + uploaded {2 * nblk * n / 1e9:.2f} GB "
         f"({'VDIF frame pairs' if frames_mode else 'payload'}) in {time.perf_counter() - t:.1f} s")
-    # plain copy of known size: the calibration kernel for the traffic counters
+    # plain copy of known size: the calibration kernel for the traffic
+    # counters, and (a larger one, timed) the HBM bandwidth a copy achieves
     calib = vfrb.DeviceArray(1 << 28, np.uint8)
     vfrb.call("vfrb_memcpy_d2d", calib.ptr, (raw if frames_mode else d0).ptr, 1 << 28, None)
     vfrb.sync()
     del calib
+    copy_gbs = measured_copy_bandwidth(vfrb)
+    log(f"device-to-device copy: {copy_gbs:.0f} GB/s (read + write)")
 
     # ---- device buffers and plans
     ring = vfrb.DeviceArray((ring_rows, nch), np.float32)
@@ -359,6 +362,7 @@ def main():
         if "bytes" in work[k]:
             e["GBps"] = work[k]["bytes"] / d / 1e9
             e["hbm_frac"] = e["GBps"] / HBM_PEAK_GBS
+            e["copy_frac"] = e["GBps"] / copy_gbs
         if "adds" in work[k]:
             e["TFLOPs"] = work[k]["adds"] / d / 1e12
         if "flops" in work[k]:
@@ -417,12 +421,28 @@ def main():
         "tile_by_storage": {k: v["tile"] for k, v in results.items()},
         "autotune_ms_by_storage": {k: v["autotune_ms"] for k, v in results.items()},
         "stages": stages_roof,
+        "hbm_copy_GBps": copy_gbs,
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": dev,
     }
     line.update(extra)
     print(json.dumps(line), flush=True)
+
+
+def measured_copy_bandwidth(vfrb, nbytes=2 << 30, reps=5):
+    """Achievable HBM bandwidth: bytes read + written per second by a
+    device-to-device copy of 2 GiB (the library's copy, hipMemcpyAsync)."""
+    a, b = vfrb.DeviceArray(nbytes, np.uint8), vfrb.DeviceArray(nbytes, np.uint8)
+    s = vfrb.Stream()
+    e0, e1 = vfrb.Event(), vfrb.Event()
+    vfrb.call("vfrb_memcpy_d2d", b.ptr, a.ptr, nbytes, s.h)
+    e0.record(s.h)
+    for _ in range(reps):
+        vfrb.call("vfrb_memcpy_d2d", b.ptr, a.ptr, nbytes, s.h)
+    e1.record(s.h)
+    e1.sync()
+    return 2.0 * nbytes * reps / (e0.elapsed_ms(e1) * 1e-3) / 1e9
 
 
 def end_to_end(vfrb, args, hraw, raw, front, back, stream, stream2, seg_data_s, i0):
