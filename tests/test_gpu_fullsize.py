@@ -12,6 +12,9 @@ properties that hold exactly whatever the size:
   and through the persistent batch kernel;
 - DM 0 is the channel mean of the window row;
 - the delay table is the oracle's bit for bit, max_delay 12 090;
+- dedisperseSTATS of non-overlapping unit impulses: every (DM, t) sees k
+  ones, so ave = k/4096 exactly and sig = sqrt((k - k^2/N) / (N(N-1))) to
+  the last bit of the float division and square root;
 - the per-DM statistics of a full output match the oracle's arithmetic;
 - the filterbank of a whole 20 s segment read as raw VDIF frame pairs (5.15 GB,
   past 2^32 bytes, with dropped pairs the device ingest zero-fills) equals the
@@ -167,3 +170,37 @@ def test_filterbank_from_frames_full_segment(gpu, full):
     fbp.execute(d0, d1, seg, b)
     assert np.array_equal(got, b.download())
     assert np.all(np.isfinite(got)) and got.min() >= 0.0
+
+
+def test_dedisperse_stats_impulses_full_size(gpu, full):
+    vfrb = gpu
+    p, dms, off, maxd, seg, rows = full
+    nt = rows - maxd
+    pulses = [(0, 0), (NDM - 1, nt - 1), (700, 4321), (1500, 12345), (333, 17500), (1200, 800)]
+    fb = np.zeros((rows, NCH), np.float32)
+    cols = np.arange(NCH)
+    for d, t0 in pulses:
+        fb[t0 + off[d], cols] += 1.0
+    assert fb.max() == 1.0  # no two impulses share a sample (checked on the delay table)
+    cnt = np.zeros(NDM * nt, np.int64)
+    dd = np.repeat(np.arange(NDM, dtype=np.int64), NCH)
+    for d, t0 in pulses:
+        t = (t0 + off[d][None, :] - off).ravel().astype(np.int64)
+        ok = (t >= 0) & (t < nt)
+        cnt += np.bincount(dd[ok] * nt + t[ok], minlength=NDM * nt)
+    k = cnt.reshape(NDM, nt).astype(np.float64)
+    del cnt, dd
+    exp_ave = (k / NCH).astype(np.float32)
+    num = (k - k * k / NCH).astype(np.float32)
+    exp_sig = np.sqrt(num / np.float32(NCH * (NCH - 1)))
+    print("[case] dedisperseSTATS impulses, full size", flush=True)
+    plan = vfrb.SearchPlan(NCH, p.tchan, p.fch1, p.chanbw, dms, rows)
+    d_fb = vfrb.DeviceArray.from_host(fb)
+    del fb
+    plan.load(d_fb, rows, bandpass=False)
+    out, sig = vfrb.DeviceArray((NDM, nt), np.float32), vfrb.DeviceArray((NDM, nt), np.float32)
+    plan.dedisperse_stats(rows, out, sig, nt)
+    assert np.array_equal(out.download(), exp_ave)
+    g = sig.download()
+    assert np.all(np.abs(g - exp_sig) <= np.spacing(np.maximum(exp_sig, np.float32(1e-30))))
+    assert np.count_nonzero(exp_sig) > 0
