@@ -22,8 +22,12 @@ def _rel_err(a, b):
     (1024, 5, 9, True),      # nave not a multiple of the FFTs in flight
     (256, 3, 4, True),
     (16384, 2, 3, True),
-    (12500, 2, 3, False),    # reference default length (mixed radix 8*5*5*5*5*... )
-    (240, 7, 5, True),       # radix 8*2*3*5
+    (12500, 2, 3, False),    # reference default length: fixed-length kernel, radices 5^5 * 4
+    (6250, 3, 4, True),      # fixed-length kernels of its halvings: 5^5 * 2
+    (2500, 4, 3, True),      # 5^4 * 4
+    (1250, 2, 5, True),      # 5^4 * 2
+    (240, 7, 5, True),       # run-time radices 8*2*3*5
+    (3000, 2, 3, True),      # run-time radices 8*5*5*3
 ])
 def test_filterbank_vs_oracle(gpu, orc, n, nave, nrows, fullband):
     vfrb = gpu

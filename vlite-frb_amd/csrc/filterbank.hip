@@ -852,13 +852,20 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
                             N, GEN_NMAX);
             return VFRB_ERR_UNSUPPORTED;
         }
-        // 12500 points (NCHAN 6250, the reference programs' default) has a
-        // kernel of its own; other lengths read their radices at run time
-        p->kernel = N == 12500
-                        ? reinterpret_cast<const void*>(&fb_fixed_kernel<12500, FIXED_BLOCK, 5, 5, 5, 5, 5, 4>)
-                        : reinterpret_cast<const void*>(&fb_generic_kernel);
+        // 12500 points (NCHAN 6250, the reference programs' default) and
+        // its halvings down to 1250 have kernels of their own (radices in
+        // factorise()'s order); other lengths read their radices at run time
+        const void* fixed = nullptr;
+        switch (N) {
+            case 12500: fixed = reinterpret_cast<const void*>(&fb_fixed_kernel<12500, FIXED_BLOCK, 5, 5, 5, 5, 5, 4>); break;
+            case 6250: fixed = reinterpret_cast<const void*>(&fb_fixed_kernel<6250, FIXED_BLOCK, 5, 5, 5, 5, 5, 2>); break;
+            case 2500: fixed = reinterpret_cast<const void*>(&fb_fixed_kernel<2500, FIXED_BLOCK, 5, 5, 5, 5, 4>); break;
+            case 1250: fixed = reinterpret_cast<const void*>(&fb_fixed_kernel<1250, FIXED_BLOCK, 5, 5, 5, 5, 2>); break;
+            default: break;
+        }
+        p->kernel = fixed ? fixed : reinterpret_cast<const void*>(&fb_generic_kernel);
         p->lds_bytes = 2 * sizeof(float) * (size_t)(N + N / 32);
-        p->block = N == 12500 ? FIXED_BLOCK : GEN_BLOCK;
+        p->block = fixed ? FIXED_BLOCK : GEN_BLOCK;
     }
     // Twiddles exp(-2*pi*i*m/N), m < N, computed in FP64 and rounded once.
     std::vector<float2> tw(N);
