@@ -555,10 +555,10 @@ struct GenPlan {
     int radix[GEN_MAXPASS];
 };
 
-template <int R>
+template <int NMAX, int R>
 __device__ __forceinline__ void gen_pass(float* __restrict__ re, float* __restrict__ im, int N, int Ns,
                                          const float2* __restrict__ tw, int tid) {
-    constexpr int MAXQ = (GEN_NMAX / R + GEN_BLOCK - 1) / GEN_BLOCK;
+    constexpr int MAXQ = (NMAX / R + GEN_BLOCK - 1) / GEN_BLOCK;
     const int nb = N / R;
     cf v[MAXQ][R];
 #pragma unroll
@@ -691,7 +691,7 @@ __device__ __forceinline__ void fixed_rest(const uint8_t* __restrict__ x0, const
 // as z = (x0-128) + i(x1-128), the complex FFT by `passes` (in LDS), then
 // |Z[k]|^2 + |Z[N-k]|^2 = |X0[k]|^2 + |X1[k]|^2 (times 2) accumulated per
 // selected channel.
-template <int BLOCK, class Passes>
+template <int BLOCK, int NMAX, class Passes>
 __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                                            int N, float* __restrict__ fb, int64_t nrows, int nave, int nskip,
                                            int nchanout, int64_t ring_rows, int64_t row0, float scale,
@@ -703,7 +703,7 @@ __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, con
     const int tid = threadIdx.x;
     const int64_t row = blockIdx.x;
     if (row >= nrows) return;
-    constexpr int CPT = (GEN_NMAX / 2 + BLOCK - 1) / BLOCK;
+    constexpr int CPT = (NMAX / 2 + BLOCK - 1) / BLOCK;  // channels per thread (N/2 at most)
     float acc[CPT];
 #pragma unroll
     for (int i = 0; i < CPT; i++) acc[i] = 0.0f;
@@ -737,12 +737,15 @@ __device__ __forceinline__ void mixed_rows(const uint8_t* __restrict__ pol0, con
 }
 
 // Any length the plan factorises: the radix sequence is read at run time.
+// NMAX >= N bounds the butterflies per thread (register arrays are sized
+// for it), so shorter lengths use a smaller bucket.
+template <int NMAX>
 __global__ void __launch_bounds__(GEN_BLOCK)
 fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol1,
                   const float2* __restrict__ tw, GenPlan plan, int N, float* __restrict__ fb,
                   int64_t nrows, int nave, int nskip, int nchanout, int64_t ring_rows, int64_t row0,
                   float scale) {
-    mixed_rows<GEN_BLOCK>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+    mixed_rows<GEN_BLOCK, NMAX>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
                [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
                    for (int e = t; e < N; e += GEN_BLOCK) {
                        const int idx = lds_pad(e);
@@ -753,11 +756,11 @@ fb_generic_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ 
                    int Ns = 1;
                    for (int p = 0; p < plan.npass; p++) {
                        switch (plan.radix[p]) {
-                           case 2: gen_pass<2>(re, im, N, Ns, tw, t); break;
-                           case 3: gen_pass<3>(re, im, N, Ns, tw, t); break;
-                           case 4: gen_pass<4>(re, im, N, Ns, tw, t); break;
-                           case 5: gen_pass<5>(re, im, N, Ns, tw, t); break;
-                           default: gen_pass<8>(re, im, N, Ns, tw, t); break;
+                           case 2: gen_pass<NMAX, 2>(re, im, N, Ns, tw, t); break;
+                           case 3: gen_pass<NMAX, 3>(re, im, N, Ns, tw, t); break;
+                           case 4: gen_pass<NMAX, 4>(re, im, N, Ns, tw, t); break;
+                           case 5: gen_pass<NMAX, 5>(re, im, N, Ns, tw, t); break;
+                           default: gen_pass<NMAX, 8>(re, im, N, Ns, tw, t); break;
                        }
                        Ns *= plan.radix[p];
                    }
@@ -773,7 +776,7 @@ fb_fixed_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ po
                 const float2* __restrict__ tw, GenPlan, int, float* __restrict__ fb, int64_t nrows, int nave,
                 int nskip, int nchanout, int64_t ring_rows, int64_t row0, float scale) {
     static_assert((R * ...) == N, "radices must multiply to N");
-    mixed_rows<BLOCK>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
+    mixed_rows<BLOCK, N>(pol0, pol1, N, fb, nrows, nave, nskip, nchanout, ring_rows, row0, scale,
                       [&](const uint8_t* x0, const uint8_t* x1, float* re, float* im, int t) {
                           fixed_rest<N, BLOCK, R...>(x0, x1, re, im, tw, t);
                       });
@@ -863,7 +866,11 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
             case 1250: fixed = reinterpret_cast<const void*>(&fb_fixed_kernel<1250, FIXED_BLOCK, 5, 5, 5, 5, 2>); break;
             default: break;
         }
-        p->kernel = fixed ? fixed : reinterpret_cast<const void*>(&fb_generic_kernel);
+        const void* generic = N <= 2048   ? reinterpret_cast<const void*>(&fb_generic_kernel<2048>)
+                              : N <= 4096 ? reinterpret_cast<const void*>(&fb_generic_kernel<4096>)
+                              : N <= 8192 ? reinterpret_cast<const void*>(&fb_generic_kernel<8192>)
+                                          : reinterpret_cast<const void*>(&fb_generic_kernel<GEN_NMAX>);
+        p->kernel = fixed ? fixed : generic;
         p->lds_bytes = 2 * sizeof(float) * (size_t)(N + N / 32);
         p->block = fixed ? FIXED_BLOCK : GEN_BLOCK;
     }
