@@ -402,14 +402,14 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
                 if (wo[m] < 0) continue;
 #pragma unroll
                 for (int p = 0; p < EPD; p++) {
-                    uint4 w;
-                    if (p == 0) {
-                        w = make_uint4(sv[m][0], sv[m][1], sv[m][2], sv[m][3]);
-                    } else {
-                        w.x = __builtin_amdgcn_alignbyte(sv[m][1], sv[m][0], p * ESZ);
-                        w.y = __builtin_amdgcn_alignbyte(sv[m][2], sv[m][1], p * ESZ);
-                        w.z = __builtin_amdgcn_alignbyte(sv[m][3], sv[m][2], p * ESZ);
-                        w.w = __builtin_amdgcn_alignbyte(sv[m][4], sv[m][3], p * ESZ);
+                    uint4 w = make_uint4(sv[m][0], sv[m][1], sv[m][2], sv[m][3]);
+                    if constexpr (EPD > 1) {  // shifted copies (then NSRC == 5)
+                        if (p > 0) {
+                            w.x = __builtin_amdgcn_alignbyte(sv[m][1], sv[m][0], p * ESZ);
+                            w.y = __builtin_amdgcn_alignbyte(sv[m][2], sv[m][1], p * ESZ);
+                            w.z = __builtin_amdgcn_alignbyte(sv[m][3], sv[m][2], p * ESZ);
+                            w.w = __builtin_amdgcn_alignbyte(sv[m][NSRC - 1], sv[m][3], p * ESZ);
+                        }
                     }
                     *reinterpret_cast<uint4*>(data + wo[m] + p * ch.stride * ESZ) = w;
                 }
