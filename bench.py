@@ -174,10 +174,12 @@ def main():
         f"({'VDIF frame pairs' if frames_mode else 'payload'}) in {time.perf_counter() - t:.1f} s")
     # plain copy of known size: the calibration kernel for the traffic
     # counters, and (a larger one, timed) the HBM bandwidth a copy achieves
-    calib = vfrb.DeviceArray(1 << 28, np.uint8)
-    vfrb.call("vfrb_memcpy_d2d", calib.ptr, (raw if frames_mode else d0).ptr, 1 << 28, None)
+    # (its own source buffer: the input of a small workload can be shorter)
+    calib_src, calib = vfrb.DeviceArray(1 << 28, np.uint8), vfrb.DeviceArray(1 << 28, np.uint8)
+    calib_src.zero()
+    vfrb.call("vfrb_memcpy_d2d", calib.ptr, calib_src.ptr, 1 << 28, None)
     vfrb.sync()
-    del calib
+    del calib, calib_src
     copy_gbs = measured_copy_bandwidth(vfrb)
     log(f"device-to-device copy: {copy_gbs:.0f} GB/s (read + write)")
 
