@@ -184,6 +184,40 @@ __device__ __forceinline__ void dft(cf* v) {
 // from the FP64-computed table and the other powers are products of those
 // (at most log2(R)-1 multiplications deep).
 template <int R>
+struct TwSet {
+    float2 w[__builtin_ctz(R)];  // w^(2^b), b < log2(R)
+};
+template <int R>
+__device__ __forceinline__ TwSet<R> load_twiddles(const float2* __restrict__ tw, int idx) {
+    TwSet<R> s;
+#pragma unroll
+    for (int b = 0; (1 << b) < R; b++) s.w[b] = tw[(1 << b) * idx];
+    return s;
+}
+// Hides a frame-invariant set's values from the optimiser once per frame, so
+// that the products built from it are not hoisted out of the frame loop into
+// registers (the asm emits no instruction).
+template <int R>
+__device__ __forceinline__ void refresh(TwSet<R>& s) {
+#pragma unroll
+    for (int b = 0; (1 << b) < R; b++) asm volatile("" : "+v"(s.w[b].x), "+v"(s.w[b].y));
+}
+// apply_twiddles below with the table values already in registers.
+template <int R>
+__device__ __forceinline__ void apply_twiddle_set(cf* v, const TwSet<R>& s) {
+    cf w[R];
+#pragma unroll
+    for (int b = 0; (1 << b) < R; b++) w[1 << b] = cf{s.w[b].x, s.w[b].y};
+#pragma unroll
+    for (int r = 3; r < R; r++) {
+        if ((r & (r - 1)) == 0) continue;
+        const int low = r & -r;
+        w[r] = cmul(w[r - low], w[low]);
+    }
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r]);
+}
+template <int R>
 __device__ __forceinline__ void apply_twiddles(cf* v, const float2* __restrict__ tw, int idx) {
     cf w[R];
 #pragma unroll
@@ -236,13 +270,15 @@ struct P2 {
 
 // One radix-R Stockham pass through LDS (NS = product of the radices
 // before it); t = thread index within its FFT, fb = element offset of that
-// FFT in LDS (a multiple of N); reads in padding PR, writes in PW.  Where
+// FFT in LDS (a multiple of N), ws = the twiddles of butterfly t mod NS
+// (the same for all of the thread's butterflies); reads in padding PR,
+// writes in PW.  Where
 // the constant part of an index cannot carry into the padding's shift
 // (checked at compile time), the address is the thread's padded base plus
 // an immediate.
 template <int N, int R, int NS, class PR, class PW>
 __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restrict__ im, int t,
-                                        int fb, const float2* __restrict__ tw) {
+                                        int fb, const TwSet<R>& ws) {
     constexpr int TPF = P2<N>::TPF;
     constexpr int Q = 32 / R;
     constexpr int STRIDE = N / R;
@@ -263,14 +299,14 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
         }
     }
     __syncthreads();
-    constexpr int tw_shift = P2<N>::LOG2 - __builtin_ctz(NS * R);  // N/(NS*R) = 1 << tw_shift
+    static_assert(TPF % NS == 0, "every butterfly group of a thread has the same twiddle index");
     const int k0 = t & (NS - 1);
     const int xw = PW::idx(fb + (t - k0) * R + k0);
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int j = t + TPF * q;
         const int k = j & (NS - 1);
-        apply_twiddles<R>(v[q], tw, k << tw_shift);
+        apply_twiddle_set<R>(v[q], ws);  // k == t mod NS for every q
         dft<R>(v[q]);
         const int base = (j - k) * R + k;
 #pragma unroll
@@ -413,14 +449,23 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             }
         }
     };
+    // The middle and last passes' twiddles depend only on the thread index:
+    // loaded once (14 complex values at 8192 points), so that inside the
+    // frame loop the next frame's bytes are the only vector loads, and no
+    // wait for a twiddle also waits for them.  Pass p's butterfly
+    // k = t mod NS_p uses w^(2^b) = tw[(2^b) * (k << s_p)], N = NS_p * R_p << s_p.
+    constexpr int NS2 = 8 * C::MID0;
+    TwSet<C::MID0> tw_m0 = load_twiddles<C::MID0>(tw, (t & 7) << (C::LOG2 - __builtin_ctz(8 * C::MID0)));
+    TwSet<(C::MID1 > 1 ? C::MID1 : 2)> tw_m1{};
+    if constexpr (C::MID1 > 1)
+        tw_m1 = load_twiddles<C::MID1>(tw, (t & (NS2 - 1)) << (C::LOG2 - __builtin_ctz(NS2 * C::MID1)));
+    TwSet<16> tw_a = load_twiddles<16>(tw, ja), tw_b = load_twiddles<16>(tw, jb);
     fetch(0);
     for (int it = 0; it < nave; it += C::FPB) {
-        // The twiddles depend only on the thread index; an opaque copy of the
-        // table pointer per frame keeps the compiler from hoisting all of them
-        // out of the loop into (spilled) registers.  The asm has no memory
-        // access: it only hides the pointer's value.
-        const float2* twf = tw;
-        asm volatile("" : "+s"(twf));
+        refresh(tw_m0);
+        if constexpr (C::MID1 > 1) refresh(tw_m1);
+        refresh(tw_a);
+        refresh(tw_b);
         // ---- pass 1: radix 8, butterflies 4t+q, inputs z[4t+q + r*N/8].
         // The offset-binary bytes enter unshifted: the radix-8 DFT reaches
         // every output but the first through differences of its inputs, which
@@ -456,10 +501,10 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         __syncthreads();
         // ---- middle passes
         if constexpr (C::MID1 > 1) {
-            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadB>(re, im, t, fbase, twf);
-            p2_pass<N, C::MID1, 8 * C::MID0, typename C::PadB, typename C::PadC>(re, im, t, fbase, twf);
+            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadB>(re, im, t, fbase, tw_m0);
+            p2_pass<N, C::MID1, 8 * C::MID0, typename C::PadB, typename C::PadC>(re, im, t, fbase, tw_m1);
         } else {
-            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadC>(re, im, t, fbase, twf);
+            p2_pass<N, C::MID0, 8, typename C::PadA, typename C::PadC>(re, im, t, fbase, tw_m0);
         }
         // last pass reads base + r*NS: NS a multiple of the padding span folds
         constexpr bool FOLD_LAST = NS % C::PadC::SPAN == 0;
@@ -474,7 +519,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                                          : C::PadC::idx(fbase + ja + r * NS);
                 a[r] = cf{re[ia], im[ia]};
             }
-            apply_twiddles<16>(a, twf, ja);
+            apply_twiddle_set<16>(a, tw_a);
             dft<16>(a);
 #pragma unroll
             for (int r = 0; r < 16; r++) na[r] = norm2(a[r]);
@@ -487,7 +532,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                                          : C::PadC::idx(fbase + jb + r * NS);
                 b[r] = cf{re[ib], im[ib]};
             }
-            apply_twiddles<16>(b, twf, jb);
+            apply_twiddle_set<16>(b, tw_b);
             dft<16>(b);
 #pragma unroll
             for (int r = 0; r < 16; r++) nb[r] = norm2(b[r]);
