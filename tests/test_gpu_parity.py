@@ -21,6 +21,7 @@ def _rel_err(a, b):
     (4096, 8, 7, False),     # MUOS cut
     (1024, 5, 9, True),      # nave not a multiple of the FFTs in flight
     (256, 3, 4, True),
+    (512, 4, 3, True),       # one radix-4 middle pass
     (16384, 2, 3, True),
     (12500, 2, 3, False),    # reference default length: fixed-length kernel, radices 5^5 * 4
     (6250, 3, 4, True),      # fixed-length kernels of its halvings: 5^5 * 2
