@@ -221,8 +221,11 @@ vfrb_status vfrb_bandpass_chan_stats(const float* fb, int32_t nrows, int32_t nch
  * carries from one call to the next, so a stream may arrive in batches, each
  * with its own table starting at slot 0.  A pair whose threads are not 0/1
  * or whose frame numbers (seconds) differ is VFRB_ERR_FORMAT: the host
- * sequencer reproduces the reference on such streams.  Synchronizes `stream`
- * (the slot count is returned to the host).
+ * sequencer reproduces the reference on such streams.  A call that fails
+ * (VFRB_ERR_FORMAT, or VFRB_ERR_CAPACITY when the slots exceed table_cap)
+ * leaves the parse state as it was: a retry with a larger table continues
+ * from the same expected frame.  Synchronizes `stream` (the slot count is
+ * returned to the host).
  */
 typedef struct vfrb_ingest vfrb_ingest;
 vfrb_status vfrb_ingest_create(vfrb_ingest** ingest, int64_t max_pairs, int32_t seconds_check);

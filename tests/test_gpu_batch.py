@@ -133,3 +133,61 @@ def test_batch_rejects_bad_jobs(gpu):
         b.set([(plan, rows + 1, out, ld)])  # more rows than loaded
     with pytest.raises(vfrb.VfrbError):
         b.dedisperse()  # no jobs set after the failure
+
+
+# DM lists at 1024 ch x 1.024 ms rows whose 64-DM blocks fit only some tiles:
+# DM step 2 is too coarse for the group tile (a group's presum rows exceed
+# the LDS budget) but fits the pair tile; DM step 4 spans more than 4 rows of
+# relative delay per channel pair, so only the per-channel tile fits; DM step
+# 0.1 fits the default group tile.
+FALLBACK_LISTS = [("P512_dm64_w16", 2.0, 64, 2200), ("T512_dm64_w16", 4.0, 64, 3600),
+                  ("G256_dm64_w16", 0.1, 256, 976)]
+
+
+def test_default_batch_walks_the_fallback_chain_on_every_set(gpu, orc):
+    """A default-tiled batch restarts from its preferred tile on every set():
+    a coarse list that moved it to the pair tile, then one that needs the
+    per-channel tile, do not keep a later fine list off the group tile."""
+    vfrb = gpu
+    names = vfrb.tile_names()
+    rng = np.random.default_rng(21)
+    batch = vfrb.Batch(vfrb.STORE_F32)
+    assert names[batch.tile] == "G256_dm64_w16"
+    keep = []
+    for want, step, ndm, rows in FALLBACK_LISTS + FALLBACK_LISTS[:1] + FALLBACK_LISTS[2:]:
+        print(f"[case] batch fallback set: {want} (DM step {step})", flush=True)
+        dms = (np.arange(ndm) * step).astype(np.float32)
+        plan = vfrb.SearchPlan(NCH, TCHAN, FCH1, CHANBW, dms, rows)
+        assert names[plan.tile] == want
+        fb = rng.normal(0, 22, (rows, NCH)).astype(np.float32)
+        d = vfrb.DeviceArray.from_host(fb)
+        plan.load(d, rows, bandpass=False)
+        nt = rows - plan.max_delay
+        out = vfrb.DeviceArray((ndm, nt), np.float32)
+        out.zero()
+        batch.set([(plan, rows, out, nt)])
+        assert names[batch.tile] == want
+        batch.dedisperse()
+        vfrb.sync()
+        got = out.download()
+        single = vfrb.DeviceArray((ndm, nt), np.float32)
+        plan.dedisperse(rows, single, nt)
+        assert np.array_equal(got, single.download())  # same tile: same summation order
+        ref = orc.dedisperse_f32(fb, plan.offsets(), nt)
+        assert np.max(np.abs(got - ref)) < 1e-4 * 22
+        keep.append((plan, rows, fb, d))
+    # a mixed set goes to the first tile that fits every job (the fine plan's
+    # tables are rebuilt for it), then the fine list alone returns to the group tile
+    print("[case] batch fallback: mixed set", flush=True)
+    mixed = [keep[2], keep[0]]
+    outs = [vfrb.DeviceArray((p.ndm, r - p.max_delay), np.float32) for p, r, _, _ in mixed]
+    batch.set([(p, r, o, r - p.max_delay) for (p, r, _, _), o in zip(mixed, outs)])
+    assert names[batch.tile] == "P512_dm64_w16"
+    batch.dedisperse()
+    vfrb.sync()
+    for (p, r, fb, _), o in zip(mixed, outs):
+        ref = orc.dedisperse_f32(fb, p.offsets(), r - p.max_delay)
+        assert np.max(np.abs(o.download() - ref)) < 1e-4 * 22
+    p, r, _, _ = keep[2]
+    batch.set([(p, r, outs[0], r - p.max_delay)])
+    assert names[batch.tile] == "G256_dm64_w16" and names[p.tile] == "G256_dm64_w16"

@@ -139,3 +139,196 @@ def test_stats_cli_matches_oracle(gpu, orc, tmp_path):
         assert np.max(np.abs(got[w, 0] - ave) / scale) < 1e-3, w
         assert np.max(np.abs(got[w, 1] / sig - 1)) < 2e-3, w
         assert np.array_equal(got[w, 2], out), w
+
+
+# A shape past the reference's launch limit at a tenth of its data rate cost:
+# 64 channels, NAVE 1 (1 us rows), 0.15 s windows: MAXNT = 150 000 rows,
+# NHALFCYC = 1172 > 1024, as at PIPELINE -d (1400) and STATS' defaults (1200).
+LIMIT_SHAPE = dict(nchan=64, nave=1, nb=50, window=0.15)
+
+
+def _limit_params(vfrb):
+    s = LIMIT_SHAPE
+    p = vfrb.params_derive(2 * s["nchan"], s["nave"], s["nb"], s["window"], fullband=True)
+    assert p.maxnt > 131072 and p.nhalfcyc > 1024
+    return p
+
+
+@pytest.mark.parametrize("always", [False, True], ids=["reference-default", "bandpass-always"])
+def test_pipeline_cli_at_reference_launch_limit(gpu, orc, tmp_path, always):
+    """By default the program reproduces the reference's output when its
+    subtractbpCAL<<<NCHANOUT,NHALFCYC>>> launch exceeds 1024 threads per
+    block (PIPELINE:323-324,658): the window is searched without bandpass.
+    --bandpass-always applies the bandpass at every shape."""
+    vfrb = gpu
+    s = LIMIT_SHAPE
+    nfft = 2 * s["nchan"]
+    p = _limit_params(vfrb)
+    dms = np.array([0.0, 0.5, 1.0], np.float32)
+    with open(tmp_path / "DM101.txt", "w") as f:
+        for d in dms:
+            f.write(f"{d:f}\n")
+    off, maxd, _ = vfrb.delay_table(dms, p.nchanout, p.fch1, p.chanbw, p.tchan)
+    ntime = p.maxnt - maxd
+    rows = p.maxnt + s["nb"]
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, rows * s["nave"] + 8, seed=12)
+    print(f"[case] PIPELINE CLI at the launch limit, maxnt={p.maxnt} nhalfcyc={p.nhalfcyc} "
+          f"bandpass_always={always}", flush=True)
+    cmd = [PIPE, "-d", "-nchan", str(s["nchan"]), "-nave", str(s["nave"]), "-nb", str(s["nb"]), "-i", "in.vdif",
+           "-o", "run", "--fullband", "--window", str(s["window"]), "--dump-window", "0"]
+    r = subprocess.run(cmd + (["--bandpass-always"] if always else []), cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + open(tmp_path / "run.log").read()
+    data = open(tmp_path / "in.vdif", "rb").read()
+    win = reference_windows(data, nfft, s["nave"], s["nb"], p.nskip, p.nchanout, p.maxnt, maxd, orc)[0]
+    bp = orc.bandpass_f64(win)
+    fil = np.fromfile(tmp_path / "run.nohead.fil", np.float32).reshape(p.maxnt, p.nchanout)
+    dd = np.fromfile(tmp_path / "OUTDD.bin", np.float32).reshape(len(dms), 1 + ntime)[:, 1:]
+    if always:
+        assert np.max(np.abs(fil - bp)) / bp.std() < 1e-3
+        ref = orc.dedisperse_f64(bp, off, ntime)
+        assert np.max(np.abs(dd - ref)) / ref.std() < 1e-3
+    else:
+        # the raw filterbank, as the reference leaves it
+        assert np.max(np.abs(fil - win)) / np.max(np.abs(win)) < 1e-5
+        ref = orc.dedisperse_f64(win, off, ntime)
+        assert np.max(np.abs(dd - ref)) / np.max(np.abs(ref)) < 1e-5
+        assert np.max(np.abs(fil - bp)) / bp.std() > 1.0  # and visibly not bandpassed
+
+
+@pytest.mark.parametrize("always", [False, True], ids=["reference-default", "bandpass-always"])
+def test_stats_cli_at_reference_launch_limit(gpu, orc, tmp_path, always):
+    """The STATS program's statistics at NHALFCYC > 1024: of the raw
+    filterbank by default (the reference's failed subtractbpCAL launch,
+    STATS:248-249,414), of the bandpassed one with --bandpass-always."""
+    vfrb = gpu
+    s = LIMIT_SHAPE
+    nfft = 2 * s["nchan"]
+    statsec = 0.01
+    p = _limit_params(vfrb)
+    tsamp = s["nave"] * nfft / 128e6
+    nblk = int(np.float32(s["window"]) / np.float32(statsec))
+    ncol = int(float(np.float32(statsec)) / tsamp)
+    rows = p.maxnt + s["nb"]
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, rows * s["nave"] + 8, seed=13)
+    print(f"[case] STATS CLI at the launch limit, maxnt={p.maxnt} nhalfcyc={p.nhalfcyc} blocks={nblk}x{ncol} "
+          f"bandpass_always={always}", flush=True)
+    cmd = [ST, "--input-file", "in.vdif", "-o", "st", "--nchan", str(s["nchan"]), "--nave", str(s["nave"]),
+           "--nb", str(s["nb"]), "--window", str(s["window"]), "--statsec", str(statsec), "--fullband"]
+    r = subprocess.run(cmd + (["--bandpass-always"] if always else []), cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + open(tmp_path / "st.log").read()
+    got = np.fromfile(tmp_path / "st.stats", np.float32).reshape(-1, 3, nblk, p.nchanout)
+    data = open(tmp_path / "in.vdif", "rb").read()
+    fb = reference_rows(data, nfft, s["nave"], s["nb"], p.nskip, p.nchanout, orc)[:p.maxnt].astype(np.float32)
+    assert got.shape[0] == 1
+    x = orc.bandpass_f32(fb) if always else fb
+    ave, sig, out = orc.chan_stats(x, ncol, nblk)
+    if always:
+        scale = np.sqrt(np.abs(sig)) + 1e-30
+        assert np.max(np.abs(got[0, 0] - ave) / scale) < 1e-3
+    else:
+        assert np.max(np.abs(got[0, 0] / ave - 1)) < 1e-5
+    assert np.max(np.abs(got[0, 1] / sig - 1)) < 2e-3
+    assert np.array_equal(got[0, 2], out)
+
+
+def _run_from_pipe(cmd, data, cwd, timeout=120):
+    """Runs cmd with `data` fed through a pipe on its stdin (--input-file -),
+    the way a shell pipeline or a ring reader hands over the VDIF stream."""
+    p = subprocess.Popen(cmd, cwd=cwd, stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE)
+    out, err = p.communicate(data, timeout=timeout)
+    return p.returncode, out.decode(errors="replace"), err.decode(errors="replace")
+
+
+def _inf_fields(path):
+    f = {}
+    for line in open(path).read().splitlines():
+        if "=" in line:
+            k, v = line.split("=", 1)
+            f[k.strip()] = v.strip()
+    return f
+
+
+def test_timeseries_cli_rotation_from_a_pipe(gpu, orc, tmp_path):
+    """vdifDADA2timeseries reading its VDIF stream from a pipe, rotating its
+    PRESTO files every 2 windows (--rotate-windows, the hourly rollover of
+    TIMESERIES:694-739 on a window count): each file set holds its windows'
+    series in order, each .inf its ntimes, and mjdstart advances by
+    ntimes*tsamp/86400 from the first frame's MJD."""
+    vfrb = gpu
+    nchan, nave, nb, window = 4096, 1, 5, 0.05
+    nfft = 2 * nchan
+    p = vfrb.params_derive(nfft, nave, nb, window, False, 3840)
+    dms = np.array([0.0, 0.3], np.float32)
+    with open(tmp_path / "dms.txt", "w") as f:
+        for d in dms:
+            f.write(f"{d:f}\n")
+    off, maxd, _ = vfrb.delay_table(dms, p.nchanout, p.fch1, p.chanbw, p.tchan)
+    ntime = p.maxnt - maxd
+    r0 = (maxd // nb) * nb
+    rows = p.maxnt + 5 * (p.maxnt - r0) + nb
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, rows * nave + 8, seed=14)
+    data = open(tmp_path / "in.vdif", "rb").read()
+    print(f"[case] TIMESERIES rotation from a pipe: maxnt={p.maxnt} max_delay={maxd} ntime={ntime}", flush=True)
+    rc, _, err = _run_from_pipe([TS, "-d", "-nchan", str(nchan), "-nave", str(nave), "-nb", str(nb), "-o", "ts",
+                                 "--input-file", "-", "--skip-pairs", "0", "--window", str(window),
+                                 "--dmlist", "dms.txt", "--rotate-windows", "2", "--final-inf"], data, tmp_path)
+    logs = "".join(open(f).read() for f in glob.glob(str(tmp_path / "ts*.log")))
+    assert rc == 0, err + logs
+    wins = reference_windows(data, nfft, nave, nb, p.nskip, p.nchanout, p.maxnt, maxd, orc)
+    nwin = len(wins)
+    assert nwin >= 6
+    tsamp = nave * nfft / 128e6
+    tstart = vfrb.vdif_dmjd(data[:32])
+    for k, dm in enumerate(dms):
+        def order(path):
+            b = os.path.basename(path)[:-4]
+            return int(b.rsplit("_w", 1)[1]) if "_w" in b else 0
+        dat = sorted((f for f in glob.glob(str(tmp_path / f"DM{dm:.1f}_*.dat")) if not f.endswith("_SIGMA.dat")),
+                     key=order)
+        # a set per 2 windows, and the empty set opened after the last rotation
+        assert len(dat) == nwin // 2 + 1, dat
+        mjd = tstart
+        w = 0
+        for i, path in enumerate(dat):
+            base = path[:-4]
+            series = np.fromfile(path, np.float32)
+            sigma = np.fromfile(base + "_SIGMA.dat", np.float32)
+            nw = min(2, nwin - w)
+            assert series.size == nw * ntime and sigma.size == nw * ntime, (k, i)
+            for j in range(nw):
+                ref = orc.dedisperse_f64(orc.bandpass_f64(wins[w + j]), off, ntime)[k]
+                got = series[j * ntime:(j + 1) * ntime]
+                assert np.max(np.abs(got - ref)) / ref.std() < 1e-3, (k, i, j)
+            inf = _inf_fields(base + ".inf")
+            assert inf["Data file name without suffix"] == os.path.basename(base)
+            assert int(inf["Number of bins in the time series"]) == nw * ntime
+            assert inf["Epoch of observation (MJD)"] == f"{mjd:.15f}", (k, i)
+            assert float(inf["Dispersion measure (cm-3 pc)"]) == pytest.approx(float(dm), abs=1e-6)
+            mjd = mjd + (nw * ntime * tsamp / 86400.0)
+            w += nw
+        assert w == nwin
+
+
+def test_stats_cli_from_a_pipe_equals_file_input(gpu, tmp_path):
+    """vdifDADA2stats with its VDIF stream on a pipe (--input-file -) writes
+    the same .stats bytes as from the file."""
+    vfrb = gpu
+    nchan, nave, nb, window, statsec = 128, 8, 5, 0.05, 0.01
+    nfft = 2 * nchan
+    p = vfrb.params_derive(nfft, nave, nb, window, fullband=True)
+    write_vdif(tmp_path / "in.vdif", vfrb, nfft, (3 * p.maxnt + nb) * nave + 8, seed=15)
+    args = ["--nchan", str(nchan), "--nave", str(nave), "--nb", str(nb), "--window", str(window), "--statsec",
+            str(statsec), "--fullband"]
+    print("[case] STATS from a file and from a pipe", flush=True)
+    r = subprocess.run([ST, "--input-file", "in.vdif", "-o", "file"] + args, cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    data = open(tmp_path / "in.vdif", "rb").read()
+    rc, _, err = _run_from_pipe([ST, "--input-file", "-", "-o", "pipe"] + args, data, tmp_path)
+    assert rc == 0, err
+    a = open(tmp_path / "file.stats", "rb").read()
+    b = open(tmp_path / "pipe.stats", "rb").read()
+    per_window = 3 * 4 * p.nchanout * int(np.float32(window) / np.float32(statsec))
+    assert len(a) % per_window == 0 and len(a) // per_window >= 2 and a == b

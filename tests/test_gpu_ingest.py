@@ -114,3 +114,36 @@ def test_filterbank_from_frames_equals_deframed(gpu):
         assert np.array_equal(a.download(), b.download())
     with pytest.raises(vfrb.VfrbError):   # rows past the table
         fbp.execute_frames(raw, table, nslots, 0, nslots * 5000 // rowb + 1, a)
+
+
+def test_ingest_failed_call_keeps_the_parse_state(gpu):
+    """A batch that fails its capacity check leaves the carried parse state
+    (expected frame) untouched: retrying it with a larger table, then the
+    next batch, gives the same payload as two good calls."""
+    vfrb = gpu
+    rng = np.random.default_rng(77)
+    npairs = 600
+    data, nkept = _stream(rng, npairs, drop=(5, 6, 7, 301, 302, 450))
+    raw = vfrb.DeviceArray.from_host(np.frombuffer(data, np.uint8))
+    print("[case] ingest capacity failure then retry", flush=True)
+    half = 300  # frame pairs in the first batch
+    ing = vfrb.Ingest(nkept)
+    small = vfrb.DeviceArray(64, np.int32)
+    with pytest.raises(vfrb.VfrbError) as e:
+        ing.frames(raw, half, small)
+    assert e.value.status == vfrb.ERR_CAPACITY
+    outs = []
+    off = 0
+    for n in (half, nkept - half):
+        table = vfrb.DeviceArray(4 * n + 64, np.int32)
+        nslots = ing.frames(raw, n, table, offset=off)
+        o = vfrb.DeviceArray(nslots * 5000, np.uint8)
+        vfrb.Ingest.gather(raw, table, nslots, 0, o, offset=off)
+        outs.append(o.download())
+        off += n * 2 * 5032
+    got = np.concatenate(outs)
+    want, _ = _device_streams(vfrb, data, nkept, False)
+    assert got.shape == want.shape and np.array_equal(got, want)
+    nbytes = want.size - 5000 - 17  # the host fill reads one pair ahead of what it copies
+    host0, _ = _host_streams(vfrb, data, False, nbytes)
+    assert np.array_equal(got[:nbytes], host0)

@@ -30,7 +30,8 @@ def _stream_bytes(vfrb, p, seed, pulse_t0):
 def _server(vfrb, p, storage, batched, **kw):
     return vfrb.Server(N, NAVE, p.nchanout, p.nskip, p.fch1, p.chanbw, p.tchan, storage=storage,
                        acc=kw.pop("acc", vfrb.ACC_F64), threshold=7.0, max_cand=8192,
-                       max_streams=kw.pop("max_streams", 8), max_segment_bytes=16 << 20,
+                       max_streams=kw.pop("max_streams", 8),
+                       max_segment_bytes=kw.pop("max_segment_bytes", 16 << 20),
                        arena_bytes=kw.pop("arena_bytes", 1 << 28), max_batch=8, batched=batched, **kw)
 
 
@@ -180,3 +181,96 @@ def test_server_admission_and_errors(gpu):
     assert sorted(r["tag"] for r in got) == [0, 1, 2, 3]
     assert all(r["status"] == 0 and r["batch_size"] == 1 for r in got)
     assert st["arena_peak_bytes"] <= arena and st["iterations"] == 4
+
+
+# Per-stream DM lists at this shape (2048-point FFT, NAVE 8: 0.128 ms rows)
+# that need different tiles: DM step 0.25 fits the pair tile but not the
+# group tile, step 0.4 only the per-channel tile, DMS the default group tile.
+TILE_LISTS = [("P512_dm64_w16", (np.arange(64) * 0.25).astype(np.float32)),
+              ("T512_dm64_w16", (np.arange(64) * 0.4).astype(np.float32)),
+              ("G256_dm64_w16", DMS)]
+LONG_ROWS = [900.5, 1100.25, 700.75, 1300.0, 650.1, 1250.6]
+
+
+def _long_stream(vfrb, p, seed):
+    nbytes = [int(r * N * NAVE) for r in LONG_ROWS]
+    total = sum(nbytes)
+    nblk = -(-total // N)
+    amp = vfrb.amp_for_snr(30, NAVE, p.nchanout)
+    spec = vfrb.synth_spec(N, seed=seed, pulses=[(3.0, 0.45, 0.002, amp)])
+    p0 = vfrb.synth_payload(spec, 0, 0, nblk)[:total]
+    p1 = vfrb.synth_payload(spec, 1, 0, nblk)[:total]
+    return p0, p1, np.concatenate([[0], np.cumsum(nbytes)])
+
+
+def _run_long(srv, streams):
+    for k in range(len(LONG_ROWS)):
+        for sid, p0, p1, cuts, base in streams:
+            a, b = cuts[k], cuts[k + 1]
+            srv.submit(sid, p0[a:b], p1[a:b], base + k)
+    srv.drain()
+    res = {}
+    while True:
+        got = srv.poll(wait_ms=0)
+        if not got:
+            break
+        for r in got:
+            res[r["tag"]] = r
+    assert len(res) == len(LONG_ROWS) * len(streams)
+    return res
+
+
+def test_server_per_stream_dm_lists_keep_their_tiles(gpu, orc):
+    """Streams whose DM lists need different dedispersion tiles share one
+    2-lane batched server.  Each iteration launches one persistent batch per
+    tile in use, so every segment runs on its own plan's tile: each stream's
+    results equal the stream alone bit for bit, and the oracle replay."""
+    vfrb = gpu
+    p = _shape(vfrb)
+    names = vfrb.tile_names()
+    for want, dms in TILE_LISTS:
+        assert names[vfrb.SearchPlan(p.nchanout, p.tchan, p.fch1, p.chanbw, dms, 4000).tile] == want
+    data = [_long_stream(vfrb, p, seed=40 + i) for i in range(len(TILE_LISTS))]
+    alone = []
+    for (want, dms), (p0, p1, cuts) in zip(TILE_LISTS, data):
+        print(f"[case] server per-stream lists: {want} alone", flush=True)
+        srv = _server(vfrb, p, vfrb.STORE_F32, batched=True, arena_bytes=1 << 30, max_segment_bytes=32 << 20)
+        sid = srv.open_stream(dms)
+        alone.append(_run_long(srv, [(sid, p0, p1, cuts, 0)]))
+        srv.close()
+    print("[case] server per-stream lists: all three in one server", flush=True)
+    srv = _server(vfrb, p, vfrb.STORE_F32, batched=True, arena_bytes=1 << 30, max_segment_bytes=32 << 20)
+    ids = [srv.open_stream(dms) for _, dms in TILE_LISTS]
+    mix = _run_long(srv, [(sid, *d, 100 * i) for i, (sid, d) in enumerate(zip(ids, data))])
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == 3 * len(LONG_ROWS)
+    assert max(r["batch_size"] for r in mix.values()) > 1, "no iteration batched two segments"
+    keys = ("status", "rows", "sample0", "nt", "ncand", "best_dm", "best_t", "best_snr")
+    arrays = ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr")
+    searched = 0
+    for i, ((want, dms), (p0, p1, cuts)) in enumerate(zip(TILE_LISTS, data)):
+        for k in range(len(LONG_ROWS)):
+            a, m = alone[i][k], mix[100 * i + k]
+            assert a["status"] == 0, (want, k, a["status"])
+            for key in keys:
+                assert a[key] == m[key], (want, k, key)
+            if a["nt"]:
+                searched += 1
+                for key in arrays:
+                    assert np.array_equal(a[key], m[key]), (want, k, key)
+        # the oracle replay of the stream's last window
+        rb = N * NAVE
+        tot = int(cuts[-1] // rb)
+        off, maxd = orc.delay_table(dms, p.nchanout, p.fch1, p.chanbw, p.tchan)
+        R = tot - int(cuts[-2] // rb)
+        W = min(tot, maxd + R)
+        r = alone[i][len(LONG_ROWS) - 1]
+        assert r["nt"] == W - maxd and r["sample0"] == tot - W
+        fb = orc.filterbank_f32(p0, p1, tot, N, NAVE, p.nskip, p.nchanout)
+        out = orc.dedisperse_f32(orc.bandpass_f32(fb[tot - W: tot]), off, W - maxd)
+        _, ave, sig, peak, pt = orc.dm_stats(out)
+        assert np.max(np.abs(r["ave"] - ave) / sig) < 1e-3
+        assert np.max(np.abs(r["sigma"] / sig - 1)) < 1e-3
+        assert np.max(np.abs(r["peak"] - peak) / np.maximum(peak, 1.0)) < 2e-3
+    assert searched >= 9

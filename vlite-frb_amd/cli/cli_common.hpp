@@ -18,6 +18,12 @@ namespace cli {
 
 inline bool eq(const char* a, const char* b) { return std::strcmp(a, b) == 0; }
 
+// Threads per block the reference's GPUs launch at most: its block-per-channel
+// subtractbpCAL<<<NCHANOUT,NHALFCYC>>> (PIPELINE:323-324,658;
+// STATS:248-249,414) fails, and leaves the filterbank as it was, when
+// NHALFCYC exceeds it.
+constexpr int kRefMaxBlockThreads = 1024;
+
 // Exits with the reference's code 2 (I/O, init) or 1 (usage/plan) after
 // printing the library's message.
 [[noreturn]] inline void die(int code, const char* what, vfrb_status st) {

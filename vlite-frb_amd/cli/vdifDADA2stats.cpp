@@ -7,7 +7,12 @@
 // CHANAVE, CHANSIGMA and CHANOUTLIER, each f32 [NSTATBLKS][NCHANOUT].  The
 // PSRDADA ring is replaced by --input-file.  Extensions (all additive):
 //   --input-file FILE, -o NAME (default vdifDADA2stats_), --nchan N, --nave N,
-//   --nb N, --window SEC (60), --statsec SEC (1), --fullband, --device N
+//   --nb N, --window SEC (60), --statsec SEC (1), --fullband, --device N,
+//   --bandpass-always: apply the bandpass at every shape.  By default the
+//   program reproduces the reference's output: its
+//   subtractbpCAL<<<NCHANOUT,NHALFCYC>>> launch (STATS:248-249,414) exceeds
+//   1024 threads per block when NHALFCYC > 1024 (the defaults give 1200) and
+//   never runs, so calcstats sees the raw filterbank.
 #include "cli_common.hpp"
 
 namespace {
@@ -19,6 +24,7 @@ struct Args {
     double window = 60.0, statsec = 1.0;
     bool fullband = false;
     int device = 0;
+    bool bandpass_always = false;
 };
 
 void print_usage() {
@@ -28,7 +34,8 @@ void print_usage() {
     std::fprintf(stderr, "  Usage:\n./vdifDADA2stats:\n");
     std::fprintf(stderr, "\t             -h:      print this help message.\n");
     std::fprintf(stderr, "\t             --input-file FILE, -o NAME, --nchan N, --nave N, --nb N, --window SEC,\n");
-    std::fprintf(stderr, "\t             --statsec SEC, --fullband, --device N   (MI355X build extensions)\n\n");
+    std::fprintf(stderr, "\t             --statsec SEC, --fullband, --device N, --bandpass-always\n");
+    std::fprintf(stderr, "\t             (MI355X build extensions)\n\n");
     std::fprintf(stderr, "  DEFAULTS:\n");
     std::fprintf(stderr, "\t             DADA_INPUT_KEY             = %x\n", 0xdada);
     std::fprintf(stderr, "\t             NAME OF OUTPUT FILE        = vdifDADA2stats.txt\n");
@@ -53,6 +60,7 @@ int parse(int argc, char** argv, Args& a) {
         else if (cli::eq(s, "--statsec")) a.statsec = std::atof(next());
         else if (cli::eq(s, "--fullband")) a.fullband = true;
         else if (cli::eq(s, "--device")) a.device = std::atoi(next());
+        else if (cli::eq(s, "--bandpass-always")) a.bandpass_always = true;
         else {
             std::fprintf(stderr, "Error! Unknown argument: %s\n\n", s);
             print_usage();
@@ -94,6 +102,10 @@ int main(int argc, char** argv) {
     const int nloopwin = p.maxnt / p.nbundle;
     std::fprintf(flog, "NFREQ   = %d\nNCHAN   = %d\nNCHANOUT= %d\nNAVE    = %d\nNBUNDLE = %d\nNLOOP   = %d\n",
                  p.nfreq, p.nchan, p.nchanout, p.nave, p.nbundle, nloopwin);
+    const bool bandpass = a.bandpass_always || p.nhalfcyc <= cli::kRefMaxBlockThreads;
+    if (!bandpass)
+        std::fprintf(flog, "NHALFCYC= %d > %d: statistics of the raw filterbank, as in the reference "
+                           "(--bandpass-always applies the bandpass)\n", p.nhalfcyc, cli::kRefMaxBlockThreads);
     if (nstatblks <= 0 || ncolstatblk <= 0 || (int64_t)ncolstatblk * (nstatblks - 1) >= p.maxnt) {
         std::fprintf(stderr, "Error! stat blocks do not fit the window\n");
         return 1;
@@ -148,8 +160,12 @@ int main(int argc, char** argv) {
         loopcnt++;
         if (loopcnt == nloopwin) {
             // STATS:414-416: subtractbpCAL then calcstats, as one pass
-            CLI_CHECK(vfrb_bandpass_chan_stats(dfb, p.maxnt, p.nchanout, ncolstatblk, nstatblks, dave, dsig, dout,
-                                               nullptr), 1);
+            if (bandpass)
+                CLI_CHECK(vfrb_bandpass_chan_stats(dfb, p.maxnt, p.nchanout, ncolstatblk, nstatblks, dave, dsig,
+                                                   dout, nullptr), 1);
+            else
+                CLI_CHECK(vfrb_chan_stats(dfb, p.maxnt, p.nchanout, ncolstatblk, nstatblks, dave, dsig, dout,
+                                          nullptr), 1);
             CLI_CHECK(vfrb_memcpy_d2h(have.data(), dave, blk_bytes, nullptr), 2);
             CLI_CHECK(vfrb_memcpy_d2h(hsig.data(), dsig, blk_bytes, nullptr), 2);
             CLI_CHECK(vfrb_memcpy_d2h(hout.data(), dout, blk_bytes, nullptr), 2);

@@ -16,6 +16,12 @@
 //                     bandpassed rows each window adds (all rows of the
 //                     first), so each input sample appears once, in order
 //   --device N
+//   --bandpass-always apply the bandpass at every shape.  By default the
+//                     program reproduces the reference's output: its
+//                     subtractbpCAL<<<NCHANOUT,NHALFCYC>>> launch
+//                     (PIPELINE:323-324,658) exceeds 1024 threads per block
+//                     when NHALFCYC > 1024 (e.g. -d: NHALFCYC 1400) and never
+//                     runs, so those windows are searched without bandpass.
 #include "cli_common.hpp"
 
 namespace {
@@ -31,6 +37,7 @@ struct Args {
     std::string cands;
     std::string fil_stream;
     int device = 0;
+    bool bandpass_always = false;
 };
 
 void print_usage() {
@@ -44,7 +51,8 @@ void print_usage() {
     std::fprintf(stderr, "\t             -nchan (int):  number of frequency channels,\n");
     std::fprintf(stderr, "\t             -nb (int):     number of bundles of size (2*2*nchan*nave) to send in 1 memcpy\n");
     std::fprintf(stderr, "\t             --fullband, --window SEC, --dmlist FILE, --storage f32|f16|u8,\n");
-    std::fprintf(stderr, "\t             --dump-window Q, --cands FILE, --fil-stream FILE, --device N\n");
+    std::fprintf(stderr, "\t             --dump-window Q, --cands FILE, --fil-stream FILE, --device N,\n");
+    std::fprintf(stderr, "\t             --bandpass-always\n");
     std::fprintf(stderr, "\t             (MI355X build extensions)\n");
     std::fprintf(stderr, "\n");
     std::fprintf(stderr, "  DEFAULTS:\n");
@@ -85,6 +93,7 @@ int parse(int argc, char** argv, Args& a) {
         else if (cli::eq(s, "--fil-stream")) a.fil_stream = next();
         else if (cli::eq(s, "--cands")) a.cands = next();
         else if (cli::eq(s, "--device")) a.device = std::atoi(next());
+        else if (cli::eq(s, "--bandpass-always")) a.bandpass_always = true;
         else if (cli::eq(s, "--storage")) {
             const char* t = next();
             if (cli::eq(t, "f32")) a.storage = VFRB_STORE_F32;
@@ -158,6 +167,10 @@ int main(int argc, char** argv) {
     std::fprintf(flog, "fch1= %lf [MHz]\n", p.fch1);
     std::fprintf(flog, "Will memcpy and filterbank time data in %g [ms] bundles.\n", p.tbundle * 1000.0);
     std::fprintf(flog, "MAXNT= %d\nNHALFCYC= %d\n", p.maxnt, p.nhalfcyc);
+    const bool bandpass = a.bandpass_always || p.nhalfcyc <= cli::kRefMaxBlockThreads;
+    if (!bandpass)
+        std::fprintf(flog, "NHALFCYC > %d: windows are not bandpassed, as in the reference "
+                           "(--bandpass-always applies it)\n", cli::kRefMaxBlockThreads);
 
     vfrb_seq* seq = nullptr;
     if (vfrb_seq_open_file(&seq, a.in.c_str()) != VFRB_OK) {
@@ -290,8 +303,9 @@ int main(int argc, char** argv) {
             fprintf(flog, "loopcnt= %d, loopcntTOT= %d  (loopcnt*NBUNDLE= %d)\n", loopcnt, loopcnt_tot,
                     loopcnt * p.nbundle);
             fprintf(flog, "executing dedisp plan...\n");
-            CLI_CHECK(vfrb_search_load(sp, dfb, 0, 0, p.maxnt, 1, nullptr), 1);
-            if (q == a.dump_window || fstream) CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
+            CLI_CHECK(vfrb_search_load(sp, dfb, 0, 0, p.maxnt, bandpass ? 1 : 0, nullptr), 1);
+            if (bandpass && (q == a.dump_window || fstream))
+                CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
             if (q == a.dump_window) CLI_CHECK(vfrb_memcpy_d2h(host_fb.data(), dfb, fb_bytes, nullptr), 2);
             if (fstream) {
                 // rows this window adds: filling restarts at (max_delay/NBUNDLE)*NBUNDLE

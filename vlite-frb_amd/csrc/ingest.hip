@@ -169,7 +169,12 @@ __global__ void __launch_bounds__(256) ingest_gather_kernel(const uint8_t* __res
 struct vfrb_ingest {
     int64_t max_pairs = 0;
     int sec_check = 0;
+    // Two parse states: [cur] is the one the next batch continues from; a
+    // batch writes its successor into [1 - cur], and the host switches only
+    // after the batch passed its format and capacity checks, so a failed
+    // call leaves the stream's state as it was.
     IngestState* d_state = nullptr;
+    int cur = 0;
     int* d_gap = nullptr;
     int64_t* d_incl = nullptr;
     int64_t* d_blk = nullptr;
@@ -206,7 +211,7 @@ vfrb_status vfrb_ingest_create(vfrb_ingest** out, int64_t max_pairs, int32_t sec
     ig->max_pairs = max_pairs;
     ig->sec_check = seconds_check != 0;
     const int64_t nblk = vfrb::ceil_div(max_pairs, kScanTile);
-    hipError_t e = hipMalloc(&ig->d_state, sizeof(IngestState));
+    hipError_t e = hipMalloc(&ig->d_state, 2 * sizeof(IngestState));
     if (e == hipSuccess) e = hipMalloc(&ig->d_gap, sizeof(int) * (size_t)max_pairs);
     if (e == hipSuccess) e = hipMalloc(&ig->d_incl, sizeof(int64_t) * (size_t)max_pairs);
     if (e == hipSuccess) e = hipMalloc(&ig->d_blk, sizeof(int64_t) * (size_t)nblk);
@@ -235,7 +240,7 @@ vfrb_status vfrb_ingest_destroy(vfrb_ingest* ig) {
 vfrb_status vfrb_ingest_reset(vfrb_ingest* ig, int32_t expected_frame, int32_t expected_second) {
     VFRB_REQUIRE(ig && expected_frame >= -1 && expected_frame < kFps, VFRB_ERR_INVALID, "bad reset");
     const IngestState st{expected_frame, expected_second};
-    VFRB_HIP(hipMemcpy(ig->d_state, &st, sizeof st, hipMemcpyHostToDevice));
+    VFRB_HIP(hipMemcpy(ig->d_state + ig->cur, &st, sizeof st, hipMemcpyHostToDevice));
     return VFRB_OK;
 }
 
@@ -249,14 +254,14 @@ vfrb_status vfrb_ingest_frames(vfrb_ingest* ig, const uint8_t* raw, int64_t npai
     const unsigned g = (unsigned)vfrb::ceil_div(npairs, 256);
     const int nblk = (int)vfrb::ceil_div(npairs, kScanTile);
     hipLaunchKernelGGL(ingest_zero_kernel, dim3(1), dim3(1), 0, s, ig->d_bad);
-    hipLaunchKernelGGL(ingest_gap_kernel, dim3(g), dim3(256), 0, s, raw, npairs, ig->sec_check, ig->d_state,
-                       ig->d_gap, ig->d_bad);
+    hipLaunchKernelGGL(ingest_gap_kernel, dim3(g), dim3(256), 0, s, raw, npairs, ig->sec_check,
+                       ig->d_state + ig->cur, ig->d_gap, ig->d_bad);
     hipLaunchKernelGGL(ingest_scan_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, s, ig->d_gap, npairs,
                        ig->d_incl, ig->d_blk);
     hipLaunchKernelGGL(ingest_blocks_kernel, dim3(1), dim3(kScanBlock), 0, s, ig->d_blk, nblk, ig->d_total);
     hipLaunchKernelGGL(ingest_clear_kernel, dim3(1024), dim3(256), 0, s, table, ig->d_total, table_cap);
     hipLaunchKernelGGL(ingest_place_kernel, dim3(g), dim3(256), 0, s, raw, npairs, ig->sec_check, ig->d_incl,
-                       ig->d_blk, table, table_cap, ig->d_state);
+                       ig->d_blk, table, table_cap, ig->d_state + (1 - ig->cur));
     VFRB_LAUNCH_CHECK("ingest kernels");
     VFRB_HIP(hipMemcpyAsync(ig->h_total, ig->d_total, sizeof(int64_t), hipMemcpyDeviceToHost, s));
     VFRB_HIP(hipMemcpyAsync(ig->h_bad, ig->d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -266,6 +271,7 @@ vfrb_status vfrb_ingest_frames(vfrb_ingest* ig, const uint8_t* raw, int64_t npai
     VFRB_REQUIRE(*ig->h_total <= table_cap, VFRB_ERR_CAPACITY, "%lld payload slots exceed the table of %lld",
                  (long long)*ig->h_total, (long long)table_cap);
     *nslots = *ig->h_total;
+    ig->cur = 1 - ig->cur;
     return VFRB_OK;
 }
 

@@ -1164,6 +1164,7 @@ struct vfrb_search_plan {
 struct vfrb_batch {
     int storage = 0, tile = -1;
     bool tile_default = false;  // created with tile_config -1: may fall back
+    int pref_tile = -1;         // default-tiled: tried first on every job set (default or autotuned)
     int max_jobs = 0, max_tiles = 0;
     int ntiles = -1;  // -1: no jobs set
     std::vector<vfrb_search_plan*> plans;
@@ -1843,7 +1844,8 @@ vfrb_status vfrb_batch_create(vfrb_batch** batch, int32_t storage, int32_t tile_
         return st;
     }
     b->tile_default = tile_config < 0;
-    const vfrb_status st = batch_configure(b, tile_config >= 0 ? tile_config : kDefaultTile);
+    b->pref_tile = tile_config >= 0 ? tile_config : kDefaultTile;
+    const vfrb_status st = batch_configure(b, b->pref_tile);
     if (st != VFRB_OK) {
         batch_free(b);
         return st;
@@ -1880,13 +1882,25 @@ vfrb_status vfrb_batch_set(vfrb_batch* b, int32_t njobs, vfrb_search_plan* const
     b->nrows.assign(nrows, nrows + njobs);
     b->outs.assign(outs, outs + njobs);
     b->ld_outs.assign(ld_outs, ld_outs + njobs);
-    vfrb_status st = batch_build(b, vfrb::as_stream(stream));
-    // a default-tiled batch steps down the fallback list until every job fits
-    for (int fi = 1; fi < (int)(sizeof(kTileFallback) / sizeof(int)) && st == VFRB_ERR_UNSUPPORTED &&
-                     b->tile_default && b->tile == kTileFallback[fi - 1];
-         fi++) {
-        st = batch_configure(b, kTileFallback[fi]);
-        if (st == VFRB_OK) st = batch_build(b, vfrb::as_stream(stream));
+    hipStream_t s = vfrb::as_stream(stream);
+    vfrb_status st = VFRB_ERR_UNSUPPORTED;
+    if (!b->tile_default) {
+        st = batch_build(b, s);
+    } else {
+        // A default-tiled batch starts every job set from its preferred tile
+        // (the default, or the autotuner's pick) and then walks the whole
+        // fallback list, so a coarse DM list in one set leaves later sets on
+        // the tile they would have had anyway.  A tile that is already
+        // configured keeps its captured graph.
+        int order[1 + sizeof(kTileFallback) / sizeof(int)];
+        int n = 0;
+        order[n++] = b->pref_tile;
+        for (int t : kTileFallback)
+            if (t != b->pref_tile) order[n++] = t;
+        for (int i = 0; i < n && st == VFRB_ERR_UNSUPPORTED; i++) {
+            st = b->tile == order[i] ? VFRB_OK : batch_configure(b, order[i]);
+            if (st == VFRB_OK) st = batch_build(b, s);
+        }
     }
     if (st != VFRB_OK) {
         b->ntiles = -1;
@@ -1961,6 +1975,7 @@ vfrb_status vfrb_batch_autotune(vfrb_batch* b, int32_t iters, float* ms, void* s
     (void)hipEventDestroy(e1);
     vfrb_status st2 = batch_configure(b, best >= 0 ? best : keep);
     if (st2 == VFRB_OK) st2 = batch_build(b, s);
+    if (st2 == VFRB_OK && best >= 0) b->pref_tile = best;
     if (st != VFRB_OK) return st;
     return st2;
 }

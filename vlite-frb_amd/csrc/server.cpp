@@ -129,7 +129,10 @@ struct Item {
 
 struct Lane {
     hipStream_t s = nullptr;
-    vfrb_batch* batch = nullptr;
+    // One batch per dedispersion tile, created on first use with that tile
+    // fixed.  A segment's jobs go to the batch of its plan's own tile, so
+    // its arithmetic (tile, summation order) is the same in any mix.
+    std::vector<vfrb_batch*> batches;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_copy = nullptr;  // this lane's host-to-device copies done (copy stream)
     bool busy = false;
@@ -273,11 +276,41 @@ void issue(vfrb_server* S, Lane& L) {
     }
     if (!jobs.empty()) {
         if (S->d.batched) {
-            vfrb_status e = vfrb_batch_set(L.batch, (int32_t)jobs.size(), plans.data(), nrows.data(),
-                                           outs.data(), lds.data(), nullptr, s);
-            if (e == VFRB_OK) e = vfrb_batch_graph_launch(L.batch, s);
-            if (e != VFRB_OK)
-                for (Item* it : jobs) it->status = e;
+            // one persistent launch per tile in use (one in the usual case:
+            // streams whose DM lists all fit the default tile)
+            std::vector<int32_t> tile(jobs.size());
+            for (size_t i = 0; i < jobs.size(); i++) tile[i] = vfrb_search_get_tile(plans[i]);
+            std::vector<bool> done(jobs.size(), false);
+            for (size_t i = 0; i < jobs.size(); i++) {
+                if (done[i]) continue;
+                const int32_t t = tile[i];
+                std::vector<size_t> grp;
+                for (size_t k = i; k < jobs.size(); k++)
+                    if (!done[k] && tile[k] == t) {
+                        grp.push_back(k);
+                        done[k] = true;
+                    }
+                vfrb_status e = VFRB_OK;
+                if (t < 0 || t >= (int32_t)L.batches.size()) e = VFRB_ERR_INVALID;
+                if (e == VFRB_OK && !L.batches[t])
+                    e = vfrb_batch_create(&L.batches[t], S->d.storage, t, S->d.max_streams, 1 << 22);
+                std::vector<vfrb_search_plan*> gp;
+                std::vector<int32_t> gr;
+                std::vector<float*> go;
+                std::vector<int64_t> gl;
+                for (size_t k : grp) {
+                    gp.push_back(plans[k]);
+                    gr.push_back(nrows[k]);
+                    go.push_back(outs[k]);
+                    gl.push_back(lds[k]);
+                }
+                if (e == VFRB_OK)
+                    e = vfrb_batch_set(L.batches[t], (int32_t)grp.size(), gp.data(), gr.data(), go.data(),
+                                       gl.data(), nullptr, s);
+                if (e == VFRB_OK) e = vfrb_batch_graph_launch(L.batches[t], s);
+                if (e != VFRB_OK)
+                    for (size_t k : grp) jobs[k]->status = e;
+            }
         } else {
             for (Item* it : jobs)
                 it->status = vfrb_search_dedisperse(it->st->plan, it->win_rows, it->st->out, it->st->ld_out, s);
@@ -462,7 +495,8 @@ void completer_loop(vfrb_server* S) {
 void server_free(vfrb_server* S) {
     for (auto& st : S->streams) free_stream(st.get());
     for (Lane& l : S->lanes) {
-        if (l.batch) vfrb_batch_destroy(l.batch);
+        for (vfrb_batch* b : l.batches)
+            if (b) vfrb_batch_destroy(b);
         if (l.ev0) (void)hipEventDestroy(l.ev0);
         if (l.ev1) (void)hipEventDestroy(l.ev1);
         if (l.ev_copy) (void)hipEventDestroy(l.ev_copy);
@@ -487,6 +521,8 @@ vfrb_status vfrb_server_create(vfrb_server** server, const vfrb_server_desc* d) 
     VFRB_REQUIRE(d->max_streams > 0 && d->max_segment_bytes > 0 && d->arena_bytes > 0 &&
                      d->max_cand >= 0, VFRB_ERR_INVALID, "bad capacities");
     VFRB_REQUIRE(d->acc == VFRB_ACC_F64 || d->acc == VFRB_ACC_F32C, VFRB_ERR_INVALID, "acc=%d", d->acc);
+    VFRB_REQUIRE(d->tile_config >= -1 && d->tile_config < vfrb_search_tile_count(), VFRB_ERR_INVALID,
+                 "tile_config=%d", d->tile_config);
     vfrb_server* S = new vfrb_server();
     S->d = *d;
     S->t0 = std::chrono::steady_clock::now();
@@ -506,8 +542,13 @@ vfrb_status vfrb_server_create(vfrb_server** server, const vfrb_server_desc* d) 
         if (e == hipSuccess) e = hipEventCreate(&l.ev0);
         if (e == hipSuccess) e = hipEventCreate(&l.ev1);
         if (e == hipSuccess && nlanes > 1) e = hipEventCreateWithFlags(&l.ev_copy, hipEventDisableTiming);
-        if (e == hipSuccess && d->batched)
-            st = vfrb_batch_create(&l.batch, d->storage, d->tile_config, d->max_streams, 1 << 22);
+        l.batches.assign(vfrb_search_tile_count(), nullptr);
+        if (e == hipSuccess && d->batched) {
+            // the batch of the configured (or default) tile up front; others on first use
+            vfrb_batch* b = nullptr;
+            st = vfrb_batch_create(&b, d->storage, d->tile_config, d->max_streams, 1 << 22);
+            if (st == VFRB_OK) l.batches[vfrb_batch_get_tile(b)] = b;
+        }
     }
     if (e != hipSuccess) st = vfrb::hip_fail(e, "vfrb_server_create", __FILE__, __LINE__);
     if (st != VFRB_OK) {
