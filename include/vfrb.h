@@ -234,6 +234,23 @@ vfrb_status vfrb_ingest_destroy(vfrb_ingest* ingest);
 vfrb_status vfrb_ingest_reset(vfrb_ingest* ingest, int32_t expected_frame, int32_t expected_second);
 vfrb_status vfrb_ingest_frames(vfrb_ingest* ingest, const uint8_t* frames, int64_t npairs,
                                int32_t* table, int64_t table_cap, int64_t* nslots, void* stream);
+/*
+ * No counterpart in the reference.  vfrb_ingest_frames for callers that
+ * cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): it allocates nothing.  Its
+ * scratch is a file-scope device array of 12.6 MB per GPU, held until the
+ * process ends; at most 2^20 frame pairs per call (larger counts are
+ * VFRB_ERR_CAPACITY).  The parse state lives with the caller:
+ * *expected_frame / *expected_second are the frame number and second
+ * expected next (-1 = accept the first pair as is) and receive the state
+ * after the batch; a failed call leaves them unchanged.  Same table and
+ * errors as vfrb_ingest_frames; synchronizes `stream`.  Calls that overlap
+ * on one GPU (several host threads or streams) share the scratch and race.
+ */
+vfrb_status vfrb_ingest_frames_noalloc(const uint8_t* frames, int64_t npairs, int32_t seconds_check,
+                                       int32_t* expected_frame, int32_t* expected_second,
+                                       int32_t* table, int64_t table_cap, int64_t* nslots,
+                                       void* stream);
 /* The de-framed payload of one pol: nslots*5000 bytes (zero slots as zeros). */
 vfrb_status vfrb_ingest_gather(const uint8_t* frames, const int32_t* table, int64_t nslots,
                                int32_t pol, uint8_t* out, void* stream);

@@ -13,6 +13,7 @@
 // error: the host sequencer (vfrb_seq_*) reproduces the reference on such
 // streams.
 #include <algorithm>
+#include <mutex>
 
 #include "common.hpp"
 
@@ -200,6 +201,58 @@ void ingest_free(vfrb_ingest* ig) {
 
 __global__ void ingest_zero_kernel(int* p) { *p = 0; }
 
+// ---------------------------------------------------------------------------
+// Ingest without allocation: the same kernels, with their scratch (gap runs,
+// scan partials, slot total, format flag, parse states) in one file-scope
+// device array instead of buffers of a vfrb_ingest.  Each GPU holds its own
+// copy until the process ends; calls on one GPU share it.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocMaxPairs = int64_t(1) << 20;  // 41 s of VLITE frame pairs
+constexpr size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+struct NoAllocTail {         // read back to the host in one copy
+    int64_t total;           // payload slots of the batch
+    int bad;                 // a malformed pair was seen
+    int pad;
+    IngestState state[2];    // [0] the caller's state, [1] the successor
+};
+
+constexpr size_t kNaGap = 0;
+constexpr size_t kNaIncl = align256(kNaGap + sizeof(int) * kNoAllocMaxPairs);
+constexpr size_t kNaBlk = align256(kNaIncl + sizeof(int64_t) * kNoAllocMaxPairs);
+constexpr size_t kNaTail = align256(kNaBlk + sizeof(int64_t) * ((kNoAllocMaxPairs + kScanTile - 1) / kScanTile));
+constexpr size_t kScratchBytes = align256(kNaTail + sizeof(NoAllocTail));  // 12.6 MB
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The device address of `scratch` on each GPU, looked up once per device
+// with that device current (host code never passes the symbol itself).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};
+
+vfrb_status scratch_of_current_device(unsigned char** out) {
+    int dev = -1;
+    VFRB_HIP(hipGetDevice(&dev));
+    VFRB_REQUIRE(dev >= 0 && dev < kMaxDevices, VFRB_ERR_UNSUPPORTED, "device %d", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* ptr = nullptr;
+        VFRB_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        VFRB_REQUIRE(ptr != nullptr, VFRB_ERR_HIP, "no device address for the ingest scratch");
+        g_scratch[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch[dev];
+    return VFRB_OK;
+}
+
+__global__ void ingest_state_set_kernel(NoAllocTail* t, int expected, int expected_sec) {
+    t->total = 0;
+    t->bad = 0;
+    t->state[0].expected = expected;
+    t->state[0].expected_sec = expected_sec;
+}
+
 }  // namespace
 
 extern "C" {
@@ -272,6 +325,52 @@ vfrb_status vfrb_ingest_frames(vfrb_ingest* ig, const uint8_t* raw, int64_t npai
                  (long long)*ig->h_total, (long long)table_cap);
     *nslots = *ig->h_total;
     ig->cur = 1 - ig->cur;
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_ingest_frames_noalloc(const uint8_t* raw, int64_t npairs, int32_t seconds_check,
+                                       int32_t* expected_frame, int32_t* expected_second, int32_t* table,
+                                       int64_t table_cap, int64_t* nslots, void* stream) {
+    VFRB_REQUIRE(raw && table && nslots && expected_frame && expected_second, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(npairs > 0, VFRB_ERR_INVALID, "npairs=%lld", (long long)npairs);
+    VFRB_REQUIRE(npairs <= kNoAllocMaxPairs, VFRB_ERR_CAPACITY,
+                 "npairs=%lld exceeds the %lld frame pairs the no-allocation scratch holds "
+                 "(vfrb_ingest_create + vfrb_ingest_frames take any count)",
+                 (long long)npairs, (long long)kNoAllocMaxPairs);
+    VFRB_REQUIRE(*expected_frame >= -1 && *expected_frame < kFps, VFRB_ERR_INVALID, "expected_frame=%d",
+                 *expected_frame);
+    VFRB_REQUIRE(((uintptr_t)raw % 4) == 0, VFRB_ERR_INVALID, "frames must be 4-byte aligned");
+    unsigned char* sp = nullptr;
+    const vfrb_status st = scratch_of_current_device(&sp);
+    if (st != VFRB_OK) return st;
+    int* gap = reinterpret_cast<int*>(sp + kNaGap);
+    int64_t* incl = reinterpret_cast<int64_t*>(sp + kNaIncl);
+    int64_t* blk = reinterpret_cast<int64_t*>(sp + kNaBlk);
+    NoAllocTail* tail = reinterpret_cast<NoAllocTail*>(sp + kNaTail);
+    hipStream_t s = vfrb::as_stream(stream);
+    const int sec = seconds_check != 0;
+    const unsigned g = (unsigned)vfrb::ceil_div(npairs, 256);
+    const int nblk = (int)vfrb::ceil_div(npairs, kScanTile);
+    // the previous call's contents are stale: total, flag and state start anew
+    hipLaunchKernelGGL(ingest_state_set_kernel, dim3(1), dim3(1), 0, s, tail, *expected_frame, *expected_second);
+    hipLaunchKernelGGL(ingest_gap_kernel, dim3(g), dim3(256), 0, s, raw, npairs, sec, &tail->state[0], gap,
+                       &tail->bad);
+    hipLaunchKernelGGL(ingest_scan_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, s, gap, npairs, incl, blk);
+    hipLaunchKernelGGL(ingest_blocks_kernel, dim3(1), dim3(kScanBlock), 0, s, blk, nblk, &tail->total);
+    hipLaunchKernelGGL(ingest_clear_kernel, dim3(1024), dim3(256), 0, s, table, &tail->total, table_cap);
+    hipLaunchKernelGGL(ingest_place_kernel, dim3(g), dim3(256), 0, s, raw, npairs, sec, incl, blk, table, table_cap,
+                       &tail->state[1]);
+    VFRB_LAUNCH_CHECK("ingest kernels (no-allocation scratch)");
+    NoAllocTail h{};
+    VFRB_HIP(hipMemcpyAsync(&h, tail, sizeof h, hipMemcpyDeviceToHost, s));
+    VFRB_HIP(hipStreamSynchronize(s));
+    VFRB_REQUIRE(h.bad == 0, VFRB_ERR_FORMAT,
+                 "a frame pair is not thread 0/1 with equal frame numbers (and seconds)");
+    VFRB_REQUIRE(h.total <= table_cap, VFRB_ERR_CAPACITY, "%lld payload slots exceed the table of %lld",
+                 (long long)h.total, (long long)table_cap);
+    *nslots = h.total;
+    *expected_frame = h.state[1].expected;
+    *expected_second = h.state[1].expected_sec;
     return VFRB_OK;
 }
 

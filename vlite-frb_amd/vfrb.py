@@ -133,6 +133,7 @@ _SIGS = {
     "vfrb_ingest_reset": (i32, [vp, i32, i32]),
     "vfrb_ingest_frames": (i32, [vp, vp, i64, vp, i64, P(i64), vp]),
     "vfrb_ingest_gather": (i32, [vp, vp, i64, i32, vp, vp]),
+    "vfrb_ingest_frames_noalloc": (i32, [vp, i64, i32, P(i32), P(i32), vp, i64, P(i64), vp]),
     "vfrb_bandpass_inplace": (i32, [vp, i32, i32, vp]),
     "vfrb_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
     "vfrb_bandpass_chan_stats": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp]),
@@ -509,6 +510,18 @@ class Ingest:
                 lib.vfrb_ingest_destroy(self.h)
         except Exception:
             pass
+
+
+def ingest_frames_noalloc(frames: DeviceArray, npairs, state, table: DeviceArray, seconds_check=False,
+                          stream=None, offset=0):
+    """vfrb_ingest_frames_noalloc: the ingest with no allocation; `state` is
+    the caller's [expected_frame, expected_second] list, updated in place on
+    success.  Returns the payload slots."""
+    ef, es, n = i32(state[0]), i32(state[1]), i64(0)
+    call("vfrb_ingest_frames_noalloc", frames.offset(offset), npairs, int(bool(seconds_check)), C.byref(ef),
+         C.byref(es), table.ptr, table.nbytes // 4, C.byref(n), _s(stream))
+    state[0], state[1] = ef.value, es.value
+    return n.value
 
 
 class SearchPlan:
