@@ -10,7 +10,8 @@ payload already resident in HBM:
     filterbank (fused unpack/FFT/detect/average) -> bandpass + pack ->
     dedispersion -> per-DM statistics -> candidate list.
 value = real-time factor (data-seconds per wall-second) of the configured
-storage; the other storages are measured in the same run and reported.
+storage (default FP32, the plain full-shape pipeline); FP16 and 8-bit
+storage are measured in the same run and reported beside it.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -53,14 +54,14 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--storage", default="f16", choices=["f32", "f16", "u8"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "u8"])
     ap.add_argument("--compare", default="f32,f16,u8", help="storages also timed in this run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tile", type=int, default=-1)
     ap.add_argument("--payload-cache", default="",
                     help="file to reuse the synthesised payload between runs (written if absent)")
-    ap.add_argument("--sections", default="c1,precision,batching,serving,stats,cli",
+    ap.add_argument("--sections", default="c1,precision,batching,serving,serving_full,stats,cli",
                     help="extra measurements reported in the same line (empty = none)")
     ap.add_argument("--input", default="frames", choices=["frames", "payload"],
                     help="device-resident input: raw VDIF frame pairs (ingested on the GPU) or de-framed payload")
@@ -201,9 +202,10 @@ def main():
     names = (["ingest"] if frames_mode else []) + ["filterbank", "bandpass_pack", "dedisperse", "dm_stats",
                                                       "candidates"]
 
-    def front(i, st, mark=None, frames=None):
+    def front(i, st, mark=None, frames=None, fb_ev=None):
         """Segment i's new rows into the ring: from the raw frame pairs
-        through the device ingest's slot table, or from the payload."""
+        through the device ingest's slot table, or from the payload.
+        fb_ev: (start, end) events around the filterbank kernel."""
         row0 = (max_delay + i * seg_rows) % ring_rows
         if frames_mode:
             # every step replays the same frames: restart the parse state
@@ -213,9 +215,12 @@ def main():
             nsl = ing.frames(fr, npairs, table, stream=st)
             if mark is not None:
                 mark.record(st)
+            fb_ev and fb_ev[0].record(st)
             fbp.execute_frames(fr, table, nsl, 0, seg_rows, ring, ring_rows=ring_rows, row0=row0, stream=st)
         else:
+            fb_ev and fb_ev[0].record(st)
             fbp.execute(d0, d1, seg_rows, ring, ring_rows=ring_rows, row0=row0, stream=st)
+        fb_ev and fb_ev[1].record(st)
 
     storages = [args.storage] + [x for x in args.compare.split(",") if x and x != args.storage]
     results = {}
@@ -270,7 +275,10 @@ def main():
         ev_f = [vfrb.Event() for _ in range(nst)]
         ev_l = [vfrb.Event() for _ in range(nst)]
         ev_e = [vfrb.Event() for _ in range(nst)]
-        spare = [vfrb.Event(), vfrb.Event()]
+        # the two heavy kernels of every timed step, on the streams they run on
+        ev_fb = [(vfrb.Event(), vfrb.Event()) for _ in range(nst)]
+        ev_dd = [vfrb.Event() for _ in range(nst)]
+        spare = vfrb.Event()
         t0 = None
         for i in range(nst):
             if i == args.warmup:
@@ -281,16 +289,25 @@ def main():
             if i > 0:
                 stream2.wait(ev_l[i - 1])
             ev_s[i].record(s2)
-            front(args.warmup + nserial + i, s2)
+            front(args.warmup + nserial + i, s2, fb_ev=ev_fb[i])
             ev_f[i].record(s2)
             stream.wait(ev_f[i])
-            back(args.warmup + nserial + i, s, [ev_l[i]] + spare)
+            back(args.warmup + nserial + i, s, [ev_l[i], ev_dd[i], spare])
             ev_e[i].record(s)
         stream.sync()
         stream2.sync()
         vfrb.sync()
         wall = time.perf_counter() - t0
         step_ms = [ev_s[i].elapsed_ms(ev_e[i]) for i in range(args.warmup, nst)]
+        # kernel durations over the timed region (pipelined: each overlaps the other stream's work)
+        timed_kernel_ms = {
+            "filterbank": float(np.mean([ev_fb[i][0].elapsed_ms(ev_fb[i][1]) for i in range(args.warmup, nst)])),
+            "dedisperse": float(np.mean([ev_l[i].elapsed_ms(ev_dd[i]) for i in range(args.warmup, nst)]))}
+        # the last step's dedispersed series and candidates (every step sees the same window)
+        out_last = out.download()
+        nc_last = min(int(ccount.download()[0]), maxc)
+        cand_last = set(zip(cdm.download()[:nc_last].tolist(), ct.download()[:nc_last].tolist()))
+        ave_last, sig_last = ave.download(), sig.download()
         ms_step = wall * 1e3 / args.steps
         rtf = args.steps * seg_data_s / wall
         serial_ms = float(np.sum([np.mean(v) for v in stage_ms.values()]))
@@ -307,7 +324,10 @@ def main():
                               serial_ms_per_step=serial_ms, serial_rtf=seg_data_s / (serial_ms * 1e-3),
                               p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                               candidates=ncand, best_dm=float(dms[best]), best_snr=float(pk[best]),
-                              tile=vfrb.tile_names()[sp.tile], autotune_ms=tune)
+                              tile=vfrb.tile_names()[sp.tile], autotune_ms=tune,
+                              timed_kernel_ms=timed_kernel_ms,
+                              snr=(out_last - ave_last[:, None]) / sig_last[:, None], cand=cand_last)
+        del out_last
         log(f"{sname}: RTF={rtf:.1f} (serial {seg_data_s / (serial_ms * 1e-3):.1f}) ms/step={ms_step:.2f} "
             f"p99={np.percentile(lat, 99):.2f} stages="
             + " ".join(f"{k}={np.mean(v):.3f}" for k, v in stage_ms.items())
@@ -330,8 +350,11 @@ def main():
         "dm_stats": dict(bytes=3.0 * ndm * seg_rows * 4),
         "candidates": dict(bytes=ndm * seg_rows * 4.0),
     }
-    dom = max(st, key=lambda k: st[k])
-    dur_s = st[dom] * 1e-3
+    # dominant kernel of the timed (pipelined) steps, timed there with HIP
+    # events on the stream it runs on
+    tk = main_r["timed_kernel_ms"]
+    dom = max(tk, key=lambda k: tk[k])
+    dur_s = tk[dom] * 1e-3
     if dom == "dedisperse":
         kind, sl, dmb, w = re.match(r"([TPG])(\d+)_dm(\d+)_w(\d+)", main_r["tile"]).groups()
         sl, dmb, w = int(sl), int(dmb), int(w)
@@ -339,24 +362,48 @@ def main():
         mode = "TPG".index(kind)  # per-channel, channel-pair, 4-channel-group tile
         kname = f"dedisp_kernel<{ts}, {sl // 64}, {dmb // w}, {w}, {mode}>"
         ach = work[dom]["adds"] / dur_s / 1e12
-        roof = dict(bound="mfma", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=ach / FP32_PEAK_TFLOPS, kernel=kname,
-                    note="no MFMA: FP32 vector-add bound; 1 add = 1 flop; adds = ndm*nch*rows_out per launch")
+        roof = dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=ach / FP32_PEAK_TFLOPS, kernel=kname, kernel_ms=tk[dom],
+                    note="no MFMA (not a contraction): FP32 vector adds and LDS reads; 1 add = 1 flop; "
+                         "adds = ndm*nch*rows_out per launch; peak = MI355X FP32 vector figure (packed FMA)")
     elif dom == "filterbank":
         # the FFT runs on the vector ALUs: its arithmetic, not its 5.5 GB of
         # HBM traffic per step, is what bounds it
         kname = f"fb_pow2_kernel<{n}, {'true' if frames_mode else 'false'}>"
         ach = work[dom]["flops"] / dur_s / 1e12
-        roof = dict(bound="mfma", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
-                    frac=ach / FP32_PEAK_TFLOPS, kernel=kname,
+        roof = dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=ach / FP32_PEAK_TFLOPS, kernel=kname, kernel_ms=tk[dom],
                     note="no MFMA: FP32 vector FFT arithmetic; flops = 2.5*N*log2(N) per real N-point FFT, "
-                         "two pols, per launch")
+                         "two pols, per launch; peak = MI355X FP32 vector figure (packed FMA)")
     else:
         kname = dom
         ach = work[dom]["bytes"] / dur_s / 1e9
         roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                     kernel=kname)
     roof["traffic"], roof["traffic_source"] = profiled_traffic(kname, args.storage, args.config)
+    # candidate lists of the other storages against this one's, on the same window
+    ref_r = results[args.storage]
+    snr_ref = ref_r["snr"]
+    near = np.abs(snr_ref / 7.0 - 1.0)
+    cand_cmp = {"threshold": 7.0, "reference_storage": args.storage,
+                "reference_candidates": len(ref_r["cand"]),
+                "reference_min_margin": float(near.min()),
+                "reference_min_margin_note": "smallest |SNR/7 - 1| over all (DM, sample) of the reference storage: "
+                                             "the relative SNR error a candidate flip needs"}
+    for k, v in results.items():
+        if k == args.storage:
+            continue
+        rel = np.abs(v["snr"] - snr_ref) / np.maximum(np.abs(snr_ref), 1.0)
+        diff = v["cand"] ^ ref_r["cand"]
+        cand_cmp[k] = {"candidates": len(v["cand"]), "identical": not diff, "only_in_one": len(diff),
+                       "max_rel_snr_error_vs_reference": float(rel.max()),
+                       "max_rel_snr_error_above_6": float(rel[snr_ref > 6.0].max()) if (snr_ref > 6.0).any()
+                       else None}
+        if diff:
+            d = np.array(sorted(diff))
+            cand_cmp[k]["differing_reference_snr"] = [float(snr_ref[a, b]) for a, b in d[:8]]
+    for v in results.values():
+        del v["snr"], v["cand"]
     stages_roof = {}
     for k, v in st.items():
         d = v * 1e-3
@@ -374,12 +421,15 @@ def main():
     # free the step's device buffers before the extra sections
     del d0, d1, raw, ring, out, cdm, ct, csnr
     extra = {}
+    ctx = {"host_payload": host_payload}
+    if "f32" in results:
+        ctx["headline_fp32_rtf"] = results["f32"]["rtf"]
     for sec in [x for x in args.sections.split(",") if x]:
         fn = {"precision": bench_precision, "batching": bench_batching, "serving": bench_serving,
-              "stats": bench_stats, "c1": bench_c1, "cli": bench_cli}[sec]
+              "serving_full": bench_serving_full, "stats": bench_stats, "c1": bench_c1, "cli": bench_cli}[sec]
         t = time.perf_counter()
         try:
-            extra[sec] = fn(vfrb, args)
+            extra[sec] = fn(vfrb, args, ctx) if sec == "serving_full" else fn(vfrb, args)
         except Exception as e:  # a failed section is reported, the headline stands
             log(f"section {sec} failed: {e!r}")
             extra[sec] = {"error": repr(e)}
@@ -401,9 +451,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.storage,
         "data": "synthetic (seeded VDIF payload: N(128,8^2) noise, 8 tones at 5 sigma, 5 dispersed pulses)",
-        "config": {"workload": W["workload"], "nchan": nch, "ndm": len(dms), "dm_max": W["dm_max"],
+        "config": {"workload": W["workload"] + f", {args.storage.upper()} storage", "nchan": nch, "ndm": len(dms), "dm_max": W["dm_max"],
                    "fft": n, "nave": nave, "tsamp_s": tchan, "segment_s": W["seg_s"],
                    "segment_rows": seg_rows, "max_delay_rows": max_delay, "window_rows": win_rows,
                    "storage": args.storage, "accumulate": "f32",
@@ -420,6 +470,8 @@ def main():
         "p99_ms_by_storage": {k: v["p99_ms"] for k, v in results.items()},
         "stage_ms_by_storage": {k: v["stage_ms"] for k, v in results.items()},
         "candidates_by_storage": {k: v["candidates"] for k, v in results.items()},
+        "candidate_lists_vs_reference_storage": cand_cmp,
+        "timed_kernel_ms_by_storage": {k: v["timed_kernel_ms"] for k, v in results.items()},
         "tile_by_storage": {k: v["tile"] for k, v in results.items()},
         "autotune_ms_by_storage": {k: v["autotune_ms"] for k, v in results.items()},
         "stages": stages_roof,
@@ -983,6 +1035,95 @@ def bench_serving(vfrb, args):
         c = res[inp]["closed_loop"]
         res[inp]["capacity_ratio_fp16_batched_vs_fp32_plain"] = (
             c["fp16_batched"]["throughput_data_s_per_s"] / c["fp32_plain"]["throughput_data_s_per_s"])
+    return res
+
+
+def bench_serving_full(vfrb, args, ctx):
+    """The serving comparison at the full shape: 4096 ch x 2048 DM (0-1000),
+    8192-point FFT, NAVE 16, 20 s segments, 8 antenna streams.  The FP16
+    continuous-batching server against the plain FP32 path (one segment at a
+    time) on the same traces: closed loop (every segment submitted at once:
+    capacity) and Poisson arrivals at 50 % and 90 % of the plain path's
+    capacity.  Input: the headline workload's payload resident in HBM, read in
+    place by the filterbank (device-resident)."""
+    host_payload = ctx["host_payload"]
+    n, nave = 8192, 16
+    p = vfrb.params_derive(n, nave, 1, 1.0, fullband=True)
+    nch, tchan = p.nchanout, p.tchan
+    dms = np.linspace(0.0, 1000.0, 2048).astype(np.float32)
+    rb = n * nave
+    seg_rows = int(20.0 / tchan)
+    nbytes = seg_rows * rb
+    assert host_payload[0].size >= nbytes
+    pools = [vfrb.DeviceArray.from_host(host_payload[pol][:nbytes]) for pol in (0, 1)]
+    nstreams, per_stream = 8, 3
+    seg_s = seg_rows * tchan
+
+    def make(batched, storage):
+        srv = vfrb.Server(n, nave, nch, p.nskip, p.fch1, p.chanbw, tchan, storage=storage,
+                          acc=vfrb.ACC_F32C if storage != vfrb.STORE_F32 else vfrb.ACC_F64,
+                          threshold=7.0, max_cand=1 << 16, max_streams=nstreams, max_segment_bytes=nbytes,
+                          arena_bytes=8 << 30, max_batch=4, batched=batched)
+        ids = [srv.open_stream(dms) for _ in range(nstreams)]
+        return srv, ids
+
+    def run(srv, ids, times):
+        """times: arrival offsets (s) of the segments, stream-interleaved; None = all at once."""
+        nseg = nstreams * per_stream
+        t0 = time.perf_counter()
+        got = []
+        for k in range(nseg):
+            if times is not None:
+                while time.perf_counter() - t0 < times[k]:
+                    got += srv.poll(wait_ms=1)
+            srv.submit_device(ids[k % nstreams], pools[0], pools[1], 0, nbytes, k, outputs=False)
+        srv.drain()
+        wall = time.perf_counter() - t0
+        got += srv.poll(wait_ms=0, max_results=4096)
+        lat = np.array([r["done_s"] - r["arrival_s"] for r in got])
+        bs = np.array([r["batch_size"] for r in got])
+        st = srv.stats()
+        return {"segments": len(got), "data_s": nseg * seg_s, "wall_s": wall,
+                "throughput_data_s_per_s": nseg * seg_s / wall,
+                "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+                "p99_latency_over_segment": float(np.percentile(lat, 99) / seg_s),
+                "mean_batch": float(bs.mean()), "iterations": st["iterations"],
+                "device_busy_s": st["device_busy_s"], "errors": int(sum(r["status"] != 0 for r in got))}
+
+    modes = (("fp16_batched", True, vfrb.STORE_F16), ("fp32_plain", False, vfrb.STORE_F32))
+    res = {"workload": f"{nstreams} streams x {per_stream} segments of {seg_s:.2f} s, 4096 ch x 2048 DM "
+                       "(0-1000), device-resident payload read in place",
+           "closed_loop": {}}
+    for name, batched, storage in modes:
+        srv, ids = make(batched, storage)
+        run(srv, ids, None)  # warm-up: plans, graphs, first-touch
+        srv.close()
+        srv, ids = make(batched, storage)
+        r = run(srv, ids, None)
+        srv.close()
+        res["closed_loop"][name] = r
+        log(f"serving 4096x2048 closed loop {name}: {r['throughput_data_s_per_s']:.1f} data-s/s, "
+            f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+    cap = res["closed_loop"]["fp32_plain"]["throughput_data_s_per_s"]
+    res["capacity_ratio_fp16_batched_vs_fp32_plain"] = (
+        res["closed_loop"]["fp16_batched"]["throughput_data_s_per_s"] / cap)
+    rng = np.random.default_rng(8)
+    for frac in (0.5, 0.9):
+        lam = frac * cap / seg_s  # segments per second
+        times = np.cumsum(rng.exponential(1.0 / lam, nstreams * per_stream))
+        for name, batched, storage in modes:
+            srv, ids = make(batched, storage)
+            r = run(srv, ids, times)
+            srv.close()
+            r["offered_data_s_per_s"] = frac * cap
+            res.setdefault(f"poisson_{int(frac * 100)}pct_of_fp32_capacity", {})[name] = r
+            log(f"serving 4096x2048 at {frac:.0%} of FP32 capacity {name}: p50 {r['p50_latency_ms']:.1f} ms, "
+                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+    if "headline_fp32_rtf" in ctx:
+        res["ratio_fp16_batched_server_vs_headline_fp32_pipeline"] = (
+            res["closed_loop"]["fp16_batched"]["throughput_data_s_per_s"] / ctx["headline_fp32_rtf"])
+    del pools
     return res
 
 

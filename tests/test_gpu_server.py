@@ -274,3 +274,62 @@ def test_server_per_stream_dm_lists_keep_their_tiles(gpu, orc):
         assert np.max(np.abs(r["sigma"] / sig - 1)) < 1e-3
         assert np.max(np.abs(r["peak"] - peak) / np.maximum(peak, 1.0)) < 2e-3
     assert searched >= 9
+
+
+def test_server_device_input_in_place_equals_copy(gpu):
+    """Device-resident segments are filterbanked in place when their rows start
+    4-byte aligned (a carried partial row is completed in the stream's carry
+    buffer first); other segments are copied into the arena.  Segments that
+    start a stream with less than a row, complete a carried row exactly, or
+    leave odd byte counts mix both paths; results equal the host-input
+    (copy) path bit for bit."""
+    vfrb = gpu
+    p = _shape(vfrb)
+    rb = N * NAVE
+    # carries: 8192, 0, 1000, 6, 9, 26, 126 (a segment short of the carried
+    # row: copied), 106, 106
+    nbytes = [rb // 2, rb // 2, 300 * rb + 1000, 250 * rb - 1000 + 6, 2 * rb + 3, 400 * rb + 17, 100, rb - 20,
+              350 * rb]
+    total = sum(nbytes)
+    amp = vfrb.amp_for_snr(30, NAVE, p.nchanout)
+    spec = vfrb.synth_spec(N, seed=9, pulses=[(2.0, 0.1, 0.002, amp)])
+    nblk = -(-total // N)
+    p0 = vfrb.synth_payload(spec, 0, 0, nblk)[:total]
+    p1 = vfrb.synth_payload(spec, 1, 0, nblk)[:total]
+    cuts = np.concatenate([[0], np.cumsum(nbytes)])
+    res = []
+    # host input; device input in place (rows 4-byte aligned); device input
+    # one byte off alignment (every segment through the arena copy)
+    for device, pad in ((False, 0), (True, 0), (True, 1)):
+        print(f"[case] server device input in place: device={device} pad={pad}", flush=True)
+        srv = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+        sid = srv.open_stream(DMS)
+        if device:
+            z = np.zeros(pad, np.uint8)
+            d0 = vfrb.DeviceArray.from_host(np.concatenate([z, p0]))
+            d1 = vfrb.DeviceArray.from_host(np.concatenate([z, p1]))
+        for k in range(len(nbytes)):
+            a, b = int(cuts[k]), int(cuts[k + 1])
+            if device:
+                srv.submit_device(sid, d0, d1, pad + a, b - a, k)
+            else:
+                srv.submit(sid, p0[a:b], p1[a:b], k)
+        srv.drain()
+        got = {}
+        while True:
+            r = srv.poll(wait_ms=0)
+            if not r:
+                break
+            got.update({x["tag"]: x for x in r})
+        srv.close()
+        res.append(got)
+    host = res[0]
+    assert sum(host[k]["nt"] > 0 for k in host) >= 3
+    for dev in res[1:]:
+        assert sorted(host) == sorted(dev) == list(range(len(nbytes)))
+        for k in host:
+            for key in ("status", "rows", "sample0", "nt", "ncand", "best_dm", "best_t", "best_snr"):
+                assert host[k][key] == dev[k][key], (k, key)
+            if host[k]["nt"]:
+                for key in ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr"):
+                    assert np.array_equal(host[k][key], dev[k][key]), (k, key)

@@ -121,6 +121,7 @@ namespace {
 struct Item {
     Request req;
     SrvStream* st = nullptr;
+    bool direct = false;  // device input read in place (no arena region)
     int64_t off[2] = {0, 0}, region = 0;
     int64_t rows = 0, sample0 = 0;
     int win_rows = 0, nt = 0;
@@ -183,6 +184,14 @@ void free_stream(SrvStream* st) {
     (void)hipHostFree(st->h_res);
 }
 
+// Device input is read in place when the filterbank's 4-byte-aligned reads
+// of the rows after the carried partial row start aligned in both pols.
+bool direct_ok(const Request& r, int64_t carry, int64_t rb) {
+    const int64_t head = carry > 0 ? rb - carry : 0;
+    if (head > r.nbytes) return false;  // the segment does not complete the row: copy path
+    return ((uintptr_t)(r.pol[0] + head) % 4) == 0 && ((uintptr_t)(r.pol[1] + head) % 4) == 0;
+}
+
 bool any_lane_free(const vfrb_server* S) {
     for (const Lane& l : S->lanes)
         if (!l.busy) return true;
@@ -228,19 +237,47 @@ void issue(vfrb_server* S, Lane& L) {
         SrvStream* st = it.st;
         const int64_t C = st->carry_bytes, tot = C + it.req.nbytes;
         const int64_t R = tot / rb, newc = tot - R * rb;
-        unsigned char* reg[2] = {S->arena_base + it.off[0], S->arena_base + it.off[1]};
         vfrb_status e = VFRB_OK;
-        for (int p = 0; p < 2 && e == VFRB_OK; p++) {
-            if (C > 0) e = vfrb_memcpy_d2d(reg[p], st->carry + p * rb, (size_t)C, s);
-            if (e == VFRB_OK && it.req.device)
-                e = vfrb_memcpy_d2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
-            else if (e == VFRB_OK && !host_copied)
-                e = vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+        if (it.direct) {
+            // Device input read in place: the carried partial row is
+            // completed in the carry buffer and filterbanked from there, the
+            // rest straight from the caller's payload; the same rows through
+            // the same kernel as the arena copy below.
+            const int64_t head = C > 0 ? std::min(rb - C, it.req.nbytes) : 0;
+            for (int p = 0; p < 2 && e == VFRB_OK && head > 0; p++)
+                e = vfrb_memcpy_d2d(st->carry + p * rb + C, it.req.pol[p], (size_t)head, s);
+            int64_t done = 0;
+            if (e == VFRB_OK && C > 0 && C + head == rb) {
+                e = vfrb_fb_execute(S->fbp, st->carry, st->carry + rb, 1, st->ring, st->ring_rows, st->total_rows,
+                                    s);
+                done = 1;
+            }
+            if (e == VFRB_OK && R > done)
+                e = vfrb_fb_execute(S->fbp, it.req.pol[0] + head, it.req.pol[1] + head, R - done, st->ring,
+                                    st->ring_rows, st->total_rows + done, s);
+            // the new partial row: the segment's last newc bytes once a row
+            // completed; otherwise (only from an empty carry) all its bytes
+            for (int p = 0; p < 2 && e == VFRB_OK; p++) {
+                if (R > 0 && newc > 0)
+                    e = vfrb_memcpy_d2d(st->carry + p * rb, it.req.pol[p] + (it.req.nbytes - newc), (size_t)newc,
+                                        s);
+                else if (R == 0 && head == 0 && it.req.nbytes > 0)
+                    e = vfrb_memcpy_d2d(st->carry + p * rb + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+            }
+        } else {
+            unsigned char* reg[2] = {S->arena_base + it.off[0], S->arena_base + it.off[1]};
+            for (int p = 0; p < 2 && e == VFRB_OK; p++) {
+                if (C > 0) e = vfrb_memcpy_d2d(reg[p], st->carry + p * rb, (size_t)C, s);
+                if (e == VFRB_OK && it.req.device)
+                    e = vfrb_memcpy_d2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+                else if (e == VFRB_OK && !host_copied)
+                    e = vfrb_memcpy_h2d(reg[p] + C, it.req.pol[p], (size_t)it.req.nbytes, s);
+            }
+            if (e == VFRB_OK && R > 0)
+                e = vfrb_fb_execute(S->fbp, reg[0], reg[1], R, st->ring, st->ring_rows, st->total_rows, s);
+            for (int p = 0; p < 2 && e == VFRB_OK; p++)
+                if (newc > 0) e = vfrb_memcpy_d2d(st->carry + p * rb, reg[p] + R * rb, (size_t)newc, s);
         }
-        if (e == VFRB_OK && R > 0)
-            e = vfrb_fb_execute(S->fbp, reg[0], reg[1], R, st->ring, st->ring_rows, st->total_rows, s);
-        for (int p = 0; p < 2 && e == VFRB_OK; p++)
-            if (newc > 0) e = vfrb_memcpy_d2d(st->carry + p * rb, reg[p] + R * rb, (size_t)newc, s);
         if (e != VFRB_OK) {
             it.status = e;
             continue;
@@ -409,14 +446,17 @@ void scheduler_loop(vfrb_server* S) {
                 ++it;
                 continue;
             }
-            const int64_t reg = align_up(st->carry_bytes + it->nbytes);
             Item item;
-            if (!S->arena.alloc(reg, &item.off[0])) break;  // arrival order: no overtaking
-            if (!S->arena.alloc(reg, &item.off[1])) {
-                S->arena.release(item.off[0], reg);
-                break;
+            item.direct = it->device && direct_ok(*it, st->carry_bytes, S->row_bytes);
+            if (!item.direct) {
+                const int64_t reg = align_up(st->carry_bytes + it->nbytes);
+                if (!S->arena.alloc(reg, &item.off[0])) break;  // arrival order: no overtaking
+                if (!S->arena.alloc(reg, &item.off[1])) {
+                    S->arena.release(item.off[0], reg);
+                    break;
+                }
+                item.region = reg;
             }
-            item.region = reg;
             item.req = *it;
             item.st = st;
             st->busy = true;
@@ -472,8 +512,10 @@ void completer_loop(vfrb_server* S) {
             }
             std::lock_guard<std::mutex> lk(S->mu);
             for (Item& it : L->items) {
-                S->arena.release(it.off[0], it.region);
-                S->arena.release(it.off[1], it.region);
+                if (it.region > 0) {
+                    S->arena.release(it.off[0], it.region);
+                    S->arena.release(it.off[1], it.region);
+                }
                 it.st->busy = false;
             }
             for (auto& r : res) S->completed.push_back(r);
