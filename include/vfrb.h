@@ -594,6 +594,10 @@ vfrb_status vfrb_seq_header(const vfrb_seq* seq, uint8_t* raw32);
 vfrb_status vfrb_seq_fill(vfrb_seq* seq, int32_t nloop, uint8_t* out0, uint8_t* out1);
 /* Frames read and frames zero-filled so far, per pol. */
 vfrb_status vfrb_seq_counters(const vfrb_seq* seq, int64_t* frames_read, int64_t* frames_filled);
+/* Input bytes the sequencer has read so far (whole 5032-byte frames).  After
+ * vfrb_seq_init the pair it starts on (the cal-aligned thread 0/1 pair of
+ * initframe01, PIPELINE:1058-1131) ends at this offset. */
+vfrb_status vfrb_seq_consumed(const vfrb_seq* seq, int64_t* bytes);
 vfrb_status vfrb_seq_close(vfrb_seq* seq);
 
 /* ------------------------------------------------------------------------ */

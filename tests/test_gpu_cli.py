@@ -332,3 +332,73 @@ def test_stats_cli_from_a_pipe_equals_file_input(gpu, tmp_path):
     b = open(tmp_path / "pipe.stats", "rb").read()
     per_window = 3 * 4 * p.nchanout * int(np.float32(window) / np.float32(statsec))
     assert len(a) % per_window == 0 and len(a) // per_window >= 2 and a == b
+
+
+def _edit_vdif(src, dst, lead_thread1=False, swap_pair=None, cut_tail=0):
+    """Copies a VDIF file of alternating thread 0/1 frames, optionally with a
+    thread-1 frame in front (misaligned start), one pair's threads swapped
+    (a stream the device ingest rejects), or the last cut_tail bytes removed."""
+    import vdif_oracle as vo
+    data = bytearray(open(src, "rb").read())
+    if swap_pair is not None:
+        o = swap_pair * 2 * 5032
+        h0, h1 = bytes(data[o:o + 32]), bytes(data[o + 5032:o + 5064])
+        data[o:o + 32], data[o + 5032:o + 5064] = h1, h0
+    if lead_thread1:
+        data = bytearray(vo.make_header(seconds=999, frame=25599, thread=1) + bytes(5000)) + data
+    if cut_tail:
+        data = data[:-cut_tail]
+    open(dst, "wb").write(bytes(data))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("case", ["drops", "offcal_misaligned", "thread_swap", "odd_bundle"])
+def test_pipeline_fast_input_equals_host_sequencer(gpu, tmp_path, case):
+    """The drop-in program's fast input path (large chunks of raw frame pairs,
+    device ingest) writes exactly the bytes of its host frame-sequencer path,
+    whose bundles the CLI oracle test pins to the reference's loop: across
+    chunk boundaries (more than 4096 frame pairs), with dropped frames,
+    a misaligned and off-cal start, a truncated last frame, a stream the
+    device ingest rejects (thread ids swapped mid-stream: the program reruns
+    on the host sequencer) and a bundle size that is not a multiple of 4
+    bytes (host path from the start)."""
+    vfrb = gpu
+    nchan, nave, nb, window = (125, 1, 1, 0.01) if case == "odd_bundle" else (128, 8, 5, 0.05)
+    nfft = 2 * nchan
+    p = vfrb.params_derive(nfft, nave, nb, window, fullband=True)
+    dms = np.array([0.0, 0.4, 0.8, 1.2], np.float32) * (0.25 if case == "odd_bundle" else 1.0)
+    rows = 9 * p.maxnt if case != "odd_bundle" else 3 * p.maxnt
+    drop = {100, 101, 102, 5000, 8191, 8192, 12000} if case == "drops" else ()
+    start = 2556 if case == "offcal_misaligned" else 0
+    write_vdif(tmp_path / "raw.vdif", vfrb, nfft, rows * nave + 64, seed=21, start_frame=start, drop=drop,
+               pulses=[] if case == "odd_bundle" else [(0.8, 0.02, 0.001, vfrb.amp_for_snr(25, nave, p.nchanout))])
+    _edit_vdif(tmp_path / "raw.vdif", tmp_path / "in.vdif", lead_thread1=case == "offcal_misaligned",
+               swap_pair=9000 if case == "thread_swap" else None, cut_tail=1234 if case == "drops" else 0)
+    size = os.path.getsize(tmp_path / "in.vdif")
+    print(f"[case] fast vs host input, {case}: {size / 2 / 5032:.0f} pairs, maxnt={p.maxnt}", flush=True)
+    outs = {}
+    for mode in ("fast", "host"):
+        d = tmp_path / mode
+        d.mkdir()
+        with open(d / "DM101.txt", "w") as f:
+            for x in dms:
+                f.write(f"{x:f}\n")
+        cmd = [PIPE, "-d", "-nchan", str(nchan), "-nave", str(nave), "-nb", str(nb), "-i", str(tmp_path / "in.vdif"),
+               "-o", "run", "--fullband", "--window", str(window), "--dump-window", "2", "--cands", "run.cands",
+               "--fil-stream", "run.fil"] + (["--host-sequencer"] if mode == "host" else [])
+        r = subprocess.run(cmd, cwd=d, capture_output=True, text=True, timeout=300)
+        log = open(d / "run.log").read()
+        assert r.returncode == 0, r.stderr + log
+        outs[mode] = {f: open(d / f, "rb").read() for f in ("OUTDD.bin", "run.nohead.fil", "run.head", "run.fil",
+                                                              "run.cands")}
+        outs[mode]["log"] = log
+    for f in ("OUTDD.bin", "run.nohead.fil", "run.head", "run.fil", "run.cands"):
+        assert outs["fast"][f] == outs["host"][f], f
+    nwin = (len(outs["host"]["run.fil"]) - len(outs["host"]["run.head"])) // (4 * p.nchanout)
+    assert nwin >= p.maxnt  # at least a window's rows went through
+    if case == "thread_swap":
+        assert "rerunning on the host frame sequencer" in outs["fast"]["log"]
+    elif case == "odd_bundle":
+        assert "bundle size not a multiple of 4 bytes" in outs["fast"]["log"]
+    else:
+        assert "fast input path:" in outs["fast"]["log"]

@@ -65,6 +65,13 @@ inline void log_to_file(const char* line, void* user) {
     if (f) std::fputs(line, f);
 }
 
+// Empties an output file opened for writing and rewinds it.
+inline void truncate(FILE* f) {
+    std::fflush(f);
+    if (ftruncate(fileno(f), 0) != 0) std::perror("ftruncate");
+    std::rewind(f);
+}
+
 inline void remove_if_exists(const char* path) {
     if (access(path, F_OK) != -1) std::remove(path);
 }

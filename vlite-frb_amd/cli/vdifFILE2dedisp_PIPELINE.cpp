@@ -22,7 +22,13 @@
 //                     (PIPELINE:323-324,658) exceeds 1024 threads per block
 //                     when NHALFCYC > 1024 (e.g. -d: NHALFCYC 1400) and never
 //                     runs, so those windows are searched without bandpass.
+//   --host-sequencer  read the input frame by frame on the host (the
+//                     reference's loop) instead of the fast path: large
+//                     chunks of raw frame pairs ingested on the GPU, with the
+//                     same bundles; streams the device ingest rejects rerun
+//                     on the host sequencer by themselves
 #include "cli_common.hpp"
+#include "fast_input.hpp"
 
 namespace {
 
@@ -38,6 +44,7 @@ struct Args {
     std::string fil_stream;
     int device = 0;
     bool bandpass_always = false;
+    bool host_sequencer = false;
 };
 
 void print_usage() {
@@ -52,7 +59,7 @@ void print_usage() {
     std::fprintf(stderr, "\t             -nb (int):     number of bundles of size (2*2*nchan*nave) to send in 1 memcpy\n");
     std::fprintf(stderr, "\t             --fullband, --window SEC, --dmlist FILE, --storage f32|f16|u8,\n");
     std::fprintf(stderr, "\t             --dump-window Q, --cands FILE, --fil-stream FILE, --device N,\n");
-    std::fprintf(stderr, "\t             --bandpass-always\n");
+    std::fprintf(stderr, "\t             --bandpass-always, --host-sequencer\n");
     std::fprintf(stderr, "\t             (MI355X build extensions)\n");
     std::fprintf(stderr, "\n");
     std::fprintf(stderr, "  DEFAULTS:\n");
@@ -94,6 +101,7 @@ int parse(int argc, char** argv, Args& a) {
         else if (cli::eq(s, "--cands")) a.cands = next();
         else if (cli::eq(s, "--device")) a.device = std::atoi(next());
         else if (cli::eq(s, "--bandpass-always")) a.bandpass_always = true;
+        else if (cli::eq(s, "--host-sequencer")) a.host_sequencer = true;
         else if (cli::eq(s, "--storage")) {
             const char* t = next();
             if (cli::eq(t, "f32")) a.storage = VFRB_STORE_F32;
@@ -285,67 +293,116 @@ int main(int argc, char** argv) {
         stream_fb.resize((size_t)p.maxnt * p.nchanout);
     }
 
-    // Main loop (PIPELINE:559-708): one bundle per iteration, a window when
-    // MAXNT/NBUNDLE bundles are in.
+    // Main loop (PIPELINE:559-708): bundles of NBUNDLE rows in stream order,
+    // a window when MAXNT/NBUNDLE bundles are in.
     const int acc = a.storage == VFRB_STORE_F32 ? VFRB_ACC_F64 : VFRB_ACC_F32C;
+    const int maxloop = p.maxnt / p.nbundle;
     int loopcnt = 0, loopcnt_tot = 0, q = 0;
-    std::fprintf(flog, "begin main loop...\n");
-    for (;;) {
+    auto process_window = [&]() {
+        // raw tail first (before the bandpass), as PIPELINE:653
+        CLI_CHECK(vfrb_memcpy_d2d(dtail, dfb + (size_t)(p.maxnt - max_delay) * p.nchanout, tail_bytes,
+                                  nullptr), 2);
+        fprintf(flog, "loopcnt= %d, loopcntTOT= %d  (loopcnt*NBUNDLE= %d)\n", loopcnt, loopcnt_tot,
+                loopcnt * p.nbundle);
+        fprintf(flog, "executing dedisp plan...\n");
+        CLI_CHECK(vfrb_search_load(sp, dfb, 0, 0, p.maxnt, bandpass ? 1 : 0, nullptr), 1);
+        if (bandpass && (q == a.dump_window || fstream))
+            CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
+        if (q == a.dump_window) CLI_CHECK(vfrb_memcpy_d2h(host_fb.data(), dfb, fb_bytes, nullptr), 2);
+        if (fstream) {
+            // rows this window adds: filling restarts at (max_delay/NBUNDLE)*NBUNDLE
+            const int r0 = q == 0 ? 0 : (max_delay / p.nbundle) * p.nbundle;
+            const size_t n = (size_t)(p.maxnt - r0) * p.nchanout;
+            CLI_CHECK(vfrb_memcpy_d2h(stream_fb.data(), dfb + (size_t)r0 * p.nchanout, n * sizeof(float),
+                                      nullptr), 2);
+            std::fwrite(stream_fb.data(), sizeof(float), n, fstream);
+        }
+        CLI_CHECK(vfrb_search_dedisperse(sp, p.maxnt, ddd, ntime, nullptr), 1);
+        CLI_CHECK(vfrb_dm_stats(ddd, ntime, ndm, ntime, acc, dfrb, dave, dsig, dpeak, dpt, nullptr), 1);
+        if (q == a.dump_window) {
+            CLI_CHECK(vfrb_memcpy_d2h(host_dd.data(), ddd, dd_bytes, nullptr), 2);
+            CLI_CHECK(vfrb_memcpy_d2h(frb.data(), dfrb, ndm * sizeof(int32_t), nullptr), 2);
+            CLI_CHECK(vfrb_memcpy_d2h(ave.data(), dave, ndm * sizeof(float), nullptr), 2);
+            CLI_CHECK(vfrb_memcpy_d2h(sig.data(), dsig, ndm * sizeof(float), nullptr), 2);
+            CLI_CHECK(vfrb_memcpy_d2h(peak.data(), dpeak, ndm * sizeof(float), nullptr), 2);
+        }
+        if (fcand) {
+            CLI_CHECK(vfrb_candidates(ddd, ntime, ndm, ntime, dave, dsig, 7.0f, dcdm, dct, dcs, maxc, dcnt,
+                                      nullptr), 1);
+            int32_t n = 0;
+            CLI_CHECK(vfrb_memcpy_d2h(&n, dcnt, sizeof n, nullptr), 2);
+            n = n < maxc ? n : maxc;
+            std::vector<int32_t> cd(n), ctt(n);
+            std::vector<float> cs(n);
+            if (n) {
+                CLI_CHECK(vfrb_memcpy_d2h(cd.data(), dcdm, n * sizeof(int32_t), nullptr), 2);
+                CLI_CHECK(vfrb_memcpy_d2h(ctt.data(), dct, n * sizeof(int32_t), nullptr), 2);
+                CLI_CHECK(vfrb_memcpy_d2h(cs.data(), dcs, n * sizeof(float), nullptr), 2);
+            }
+            // the kernel appends in completion order: written sorted by (DM, sample)
+            std::vector<int> idx(n);
+            for (int k = 0; k < n; k++) idx[k] = k;
+            std::sort(idx.begin(), idx.end(),
+                      [&](int x, int y) { return cd[x] != cd[y] ? cd[x] < cd[y] : ctt[x] < ctt[y]; });
+            for (int k : idx)
+                std::fprintf(fcand, "%d %d %.3f %d %.4f\n", q, cd[k], dms[cd[k]], ctt[k], cs[k]);
+        }
+        // restore the tail to the front; refill from max_delay/NBUNDLE bundles
+        CLI_CHECK(vfrb_memcpy_d2d(dfb, dtail, tail_bytes, nullptr), 2);
+        loopcnt = max_delay / p.nbundle;
+        q++;
+    };
+    // nb consecutive bundles of the payload stream, resident on the device:
+    // one filterbank launch per run of bundles inside one window
+    auto consume = [&](const uint8_t* p0, const uint8_t* p1, int64_t nb) {
+        int64_t done = 0;
+        while (done < nb) {
+            const int64_t take = std::min<int64_t>(nb - done, maxloop - loopcnt);
+            CLI_CHECK(vfrb_fb_execute(fbp, p0 + done * (int64_t)nloop, p1 + done * (int64_t)nloop,
+                                      take * p.nbundle, dfb, 0, (int64_t)loopcnt * p.nbundle, nullptr), 1);
+            loopcnt += (int)take;
+            loopcnt_tot += (int)take;
+            done += take;
+            if (loopcnt == maxloop) process_window();
+        }
+    };
+    std::fprintf(flog, "begin main loop... (set-up %.3f s)\n", cli::now_s() - t_start);
+    bool fast = !a.host_sequencer;
+    if (fast) {
+        std::string why;
+        const int r = cli::fast_frames(a.in.c_str(), seq, (int64_t)nloop, consume, flog, &why);
+        if (r == cli::kFastFallback) {
+            // the reference's frame loop handles what the device ingest
+            // rejects: start over on the host sequencer, outputs truncated
+            std::fprintf(flog, "fast input path stopped (%s): rerunning on the host frame sequencer\n",
+                         why.c_str());
+            vfrb_seq_close(seq);
+            if (vfrb_seq_open_file(&seq, a.in.c_str()) != VFRB_OK) {
+                std::fprintf(stderr, "Unable to open input VDIF file: %s\n", a.in.c_str());
+                return 2;
+            }
+            vfrb_seq_set_log(seq, cli::log_to_file, flog);
+            if (vfrb_seq_init(seq) != VFRB_OK) return 2;
+            if (fcand) {
+                cli::truncate(fcand);
+                std::fprintf(fcand, "# window dm_index dm sample snr\n");
+            }
+            if (fstream) {
+                cli::truncate(fstream);
+                std::fwrite(hbuf, 1, hlen, fstream);
+            }
+            CLI_CHECK(vfrb_memset(dfb, 0, fb_bytes, nullptr), 2);
+            loopcnt = loopcnt_tot = q = 0;
+            fast = false;
+        } else if (r != 0) {
+            return 2;
+        }
+    }
+    while (!fast) {
         if (vfrb_seq_fill(seq, (int32_t)nloop, h0, h1) != VFRB_OK) break;
         CLI_CHECK(vfrb_memcpy_h2d(d0, h0, nloop, nullptr), 2);
         CLI_CHECK(vfrb_memcpy_h2d(d1, h1, nloop, nullptr), 2);
-        CLI_CHECK(vfrb_fb_execute(fbp, d0, d1, p.nbundle, dfb, 0, (int64_t)loopcnt * p.nbundle, nullptr), 1);
-        loopcnt++;
-        if (loopcnt == p.maxnt / p.nbundle) {
-            // raw tail first (before the bandpass), as PIPELINE:653
-            CLI_CHECK(vfrb_memcpy_d2d(dtail, dfb + (size_t)(p.maxnt - max_delay) * p.nchanout, tail_bytes,
-                                      nullptr), 2);
-            fprintf(flog, "loopcnt= %d, loopcntTOT= %d  (loopcnt*NBUNDLE= %d)\n", loopcnt, loopcnt_tot,
-                    loopcnt * p.nbundle);
-            fprintf(flog, "executing dedisp plan...\n");
-            CLI_CHECK(vfrb_search_load(sp, dfb, 0, 0, p.maxnt, bandpass ? 1 : 0, nullptr), 1);
-            if (bandpass && (q == a.dump_window || fstream))
-                CLI_CHECK(vfrb_bandpass_inplace(dfb, p.maxnt, p.nchanout, nullptr), 1);
-            if (q == a.dump_window) CLI_CHECK(vfrb_memcpy_d2h(host_fb.data(), dfb, fb_bytes, nullptr), 2);
-            if (fstream) {
-                // rows this window adds: filling restarts at (max_delay/NBUNDLE)*NBUNDLE
-                const int r0 = q == 0 ? 0 : (max_delay / p.nbundle) * p.nbundle;
-                const size_t n = (size_t)(p.maxnt - r0) * p.nchanout;
-                CLI_CHECK(vfrb_memcpy_d2h(stream_fb.data(), dfb + (size_t)r0 * p.nchanout, n * sizeof(float),
-                                          nullptr), 2);
-                std::fwrite(stream_fb.data(), sizeof(float), n, fstream);
-            }
-            CLI_CHECK(vfrb_search_dedisperse(sp, p.maxnt, ddd, ntime, nullptr), 1);
-            CLI_CHECK(vfrb_dm_stats(ddd, ntime, ndm, ntime, acc, dfrb, dave, dsig, dpeak, dpt, nullptr), 1);
-            if (q == a.dump_window) {
-                CLI_CHECK(vfrb_memcpy_d2h(host_dd.data(), ddd, dd_bytes, nullptr), 2);
-                CLI_CHECK(vfrb_memcpy_d2h(frb.data(), dfrb, ndm * sizeof(int32_t), nullptr), 2);
-                CLI_CHECK(vfrb_memcpy_d2h(ave.data(), dave, ndm * sizeof(float), nullptr), 2);
-                CLI_CHECK(vfrb_memcpy_d2h(sig.data(), dsig, ndm * sizeof(float), nullptr), 2);
-                CLI_CHECK(vfrb_memcpy_d2h(peak.data(), dpeak, ndm * sizeof(float), nullptr), 2);
-            }
-            if (fcand) {
-                CLI_CHECK(vfrb_candidates(ddd, ntime, ndm, ntime, dave, dsig, 7.0f, dcdm, dct, dcs, maxc, dcnt,
-                                          nullptr), 1);
-                int32_t n = 0;
-                CLI_CHECK(vfrb_memcpy_d2h(&n, dcnt, sizeof n, nullptr), 2);
-                n = n < maxc ? n : maxc;
-                std::vector<int32_t> cd(n), ctt(n);
-                std::vector<float> cs(n);
-                if (n) {
-                    CLI_CHECK(vfrb_memcpy_d2h(cd.data(), dcdm, n * sizeof(int32_t), nullptr), 2);
-                    CLI_CHECK(vfrb_memcpy_d2h(ctt.data(), dct, n * sizeof(int32_t), nullptr), 2);
-                    CLI_CHECK(vfrb_memcpy_d2h(cs.data(), dcs, n * sizeof(float), nullptr), 2);
-                }
-                for (int k = 0; k < n; k++)
-                    std::fprintf(fcand, "%d %d %.3f %d %.4f\n", q, cd[k], dms[cd[k]], ctt[k], cs[k]);
-            }
-            // restore the tail to the front; refill from max_delay/NBUNDLE bundles
-            CLI_CHECK(vfrb_memcpy_d2d(dfb, dtail, tail_bytes, nullptr), 2);
-            loopcnt = max_delay / p.nbundle;
-            q++;
-        }
-        loopcnt_tot++;
+        consume(d0, d1, 1);
     }
     std::fprintf(flog, "Looped %d total times, (loopcnt= %d) (q= %d)\n\n", loopcnt_tot, loopcnt, q);
 

@@ -373,6 +373,12 @@ vfrb_status vfrb_seq_fill(vfrb_seq* s, int32_t nloop, uint8_t* out0, uint8_t* ou
     return VFRB_OK;
 }
 
+vfrb_status vfrb_seq_consumed(const vfrb_seq* s, int64_t* bytes) {
+    VFRB_REQUIRE(s && bytes, VFRB_ERR_INVALID, "NULL argument");
+    *bytes = s->frames_read * (int64_t)VFRB_FRAMELEN;
+    return VFRB_OK;
+}
+
 vfrb_status vfrb_seq_counters(const vfrb_seq* s, int64_t* frames_read, int64_t* frames_filled) {
     VFRB_REQUIRE(s, VFRB_ERR_INVALID, "NULL sequencer");
     if (frames_read) *frames_read = s->frames_read;

@@ -183,6 +183,7 @@ _SIGS = {
     "vfrb_seq_header": (i32, [vp, P(C.c_uint8)]),
     "vfrb_seq_fill": (i32, [vp, i32, P(C.c_uint8), P(C.c_uint8)]),
     "vfrb_seq_counters": (i32, [vp, P(i64), P(i64)]),
+    "vfrb_seq_consumed": (i32, [vp, P(i64)]),
     "vfrb_seq_close": (i32, [vp]),
     "vfrb_sigproc_header": (i32, [P(C.c_uint8), i32, P(i32), f64, f64, i32, i32, f64, f64]),
     "vfrb_presto_inf": (i32, [C.c_char_p, i32, C.c_char_p, f64, i32, f64, f32, f64, f64, i32, f64]),
