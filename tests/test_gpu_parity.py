@@ -345,12 +345,13 @@ def test_dedisperse_stats_variant(gpu, orc):
     assert np.allclose(sig.download(), rsig, rtol=1e-3)
 
 
-@pytest.mark.parametrize("mean,rtol", [(0.0, 2e-5), (40.0, 2e-3)])
+@pytest.mark.parametrize("mean,rtol", [(0.0, 2e-5), (40.0, 2e-5), (-300.0, 2e-5)])
 def test_dedisperse_stats_tiles(gpu, orc, mean, rtol):
     """The tiled dedisperseSTATS over several DM blocks and time tiles, a
     window longer than one tile row and a ragged last tile: ave within float
-    summation order of the oracle; sigma from one-pass sums, whose
-    cancellation grows with mean^2/variance (exact to ~1e-5 at mean 0, the
+    summation order of the oracle; sigma from one-pass sums of x - K, with K
+    the window's level (0 for a centred window), so they do not cancel: the
+    same accuracy at mean 40 and -300 (not bandpassed) as at mean 0 (the
     bandpassed window the programs feed it)."""
     vfrb = gpu
     nch, tchan = 1000, 4 * 12500 / 128e6

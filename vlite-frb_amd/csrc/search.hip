@@ -290,7 +290,7 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
             const float* __restrict__ chan_scale,  // u8 only
             const int* __restrict__ f16_exp, const float* __restrict__ offsum,
             float* __restrict__ out, int64_t ld_out, int b, int tx, unsigned char* lds_raw,
-            float* __restrict__ sig = nullptr) {
+            float* __restrict__ sig = nullptr, const float* __restrict__ shift = nullptr) {
     // LDS: [kChunkMax][DMB] u32 byte offsets of each (channel, DM) read,
     // then the staged rows.  A row holds COPIES copies of the channel's
     // samples, copy p shifted by p elements, so that the dword a lane reads
@@ -318,6 +318,10 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
     const uint32_t ldw = (uint32_t)(ld / EPD);  // dwords per window row
 
     static_assert(!ST || ESZ == 4, "dedisperseSTATS reads an f32 window");
+    // dedisperseSTATS: a window-wide shift K (0 unless the window's level is
+    // large against its spread) is subtracted from every staged sample, so
+    // the one-pass sums S1 = sum(x - K), S2 = sum((x - K)^2) do not cancel.
+    const float kshift = ST ? shift[0] : 0.0f;
     float acc[DW][SL];
     float acc2[ST ? DW : 1][ST ? SL : 1];
 #pragma unroll
@@ -403,6 +407,12 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
 #pragma unroll
                 for (int p = 0; p < EPD; p++) {
                     uint4 w = make_uint4(sv[m][0], sv[m][1], sv[m][2], sv[m][3]);
+                    if constexpr (ST) {  // dedisperseSTATS sums x - K (see the epilogue)
+                        w.x = __float_as_uint(__uint_as_float(w.x) - kshift);
+                        w.y = __float_as_uint(__uint_as_float(w.y) - kshift);
+                        w.z = __float_as_uint(__uint_as_float(w.z) - kshift);
+                        w.w = __float_as_uint(__uint_as_float(w.w) - kshift);
+                    }
                     if constexpr (EPD > 1) {  // shifted copies (then NSRC == 5)
                         if (p > 0) {
                             w.x = __builtin_amdgcn_alignbyte(sv[m][1], sv[m][0], p * ESZ);
@@ -476,18 +486,21 @@ dedisp_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, 
                 const int t = t0 + EPD * lane + 64 * EPD * q + e;
                 const float a = acc[d][EPD * q + e];
                 if (t < nt) {
-                    if constexpr (ESZ == 4) o[t] = a / fnch;
+                    if constexpr (ST) o[t] = a / fnch + kshift;
+                    else if constexpr (ESZ == 4) o[t] = a / fnch;
                     else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
                     else o[t] = (a + add) / fnch;
                     if constexpr (ST) {
-                        // dedisperseSTATS (TIMESERIES:1462-1477): ave = S1/N in
+                        // dedisperseSTATS (TIMESERIES:1462-1477): ave = sum(x)/N in
                         // float, sum = sum(x - ave), sum2 = sum((x - ave)^2),
-                        // sig = sqrt((sum2 - sum*sum) / (N(N-1))).  sum and sum2
-                        // follow from S1 and S2 = sum(x^2) exactly in real
-                        // arithmetic; they are formed in double.
-                        const double ave = (double)(a / fnch), s1 = a, n = fnch;
-                        const double sum = s1 - n * ave;
-                        const double sum2 = (double)acc2[d][EPD * q + e] - 2.0 * ave * s1 + n * ave * ave;
+                        // sig = sqrt((sum2 - sum*sum) / (N(N-1))).  With the
+                        // shifted sums S1 = sum(x - K), S2 = sum((x - K)^2) and
+                        // d = ave - K, sum = S1 - N d and sum2 = S2 - 2 d S1 + N d^2
+                        // exactly in real arithmetic; they are formed in double.
+                        const double ave = (double)o[t], s1 = a, n = fnch;
+                        const double dd = ave - (double)kshift;
+                        const double sum = s1 - n * dd;
+                        const double sum2 = (double)acc2[d][EPD * q + e] - 2.0 * dd * s1 + n * dd * dd;
                         const float num = fmaxf((float)(sum2 - sum * sum), 0.0f);
                         sig[(int64_t)dm * ld_out + t] = sqrtf(num / (float)(nch * (nch - 1)));
                     }
@@ -1054,12 +1067,51 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(Wav
 dedisp_stats_kernel(const float* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
                     const uint32_t* __restrict__ roffs, const int* __restrict__ dmin,
                     const Chunk* __restrict__ chunks, const int2* __restrict__ blk_chunks,
-                    float* __restrict__ out, float* __restrict__ sig, int64_t ld_out, int nblk) {
+                    float* __restrict__ out, float* __restrict__ sig, int64_t ld_out, int nblk,
+                    const float* __restrict__ shift) {
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int id = xcd_tile(blockIdx.x, gridDim.x);
     const int b = id % nblk, tx = id / nblk;
     dedisp_tile<float, SL, DW, W, true>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks, nullptr,
-                                        nullptr, nullptr, out, ld_out, b, tx, lds_raw, sig);
+                                        nullptr, nullptr, out, ld_out, b, tx, lds_raw, sig, shift);
+}
+
+// The dedisperseSTATS shift K of a channel-major f32 window, from three rows
+// (first, middle, last) of every channel: their mean m rounded to 12
+// significant bits when |m| exceeds twice their spread, else 0.  A bandpassed
+// (zero-mean) window keeps K = 0, i.e. plain sums; a window with a large
+// level gets sums of x - K that do not cancel.
+__global__ void __launch_bounds__(256) stats_shift_kernel(const float* __restrict__ win, int64_t ld, int nch,
+                                                          int nrows, float* __restrict__ shift) {
+    __shared__ double s1[256], s2[256];
+    double a = 0.0, b = 0.0;
+    const int64_t rows[3] = {0, nrows / 2, nrows - 1};
+    for (int c = threadIdx.x; c < nch; c += 256)
+        for (int k = 0; k < 3; k++) {
+            const double x = win[(int64_t)c * ld + rows[k]];
+            a += x;
+            b += x * x;
+        }
+    s1[threadIdx.x] = a;
+    s2[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            s1[threadIdx.x] += s1[threadIdx.x + h];
+            s2[threadIdx.x] += s2[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double n = 3.0 * nch, m = s1[0] / n;
+        const double sd = sqrt(fmax(s2[0] / n - m * m, 0.0));
+        float k = 0.0f;
+        if (isfinite(m) && fabs(m) > 2.0 * sd && m != 0.0) {
+            const int e = ilogb(m);
+            k = (float)ldexp(rint(ldexp(m, 11 - e)), e - 11);
+        }
+        shift[0] = k;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,6 +1208,7 @@ struct vfrb_search_plan {
     float* d_u8_off = nullptr;
     float* d_u8_offsum = nullptr;
     float* d_tmp_fb = nullptr;  // time-major staging for vfrb_search_execute
+    float* d_shift = nullptr;   // dedisperseSTATS window shift, allocated with its tables
     TileSet ts;   // tables of the dedispersion tile in use
     TileSet sts;  // tables of the dedisperseSTATS tile, built on first use
     int loaded_rows = 0;
@@ -1438,6 +1491,7 @@ void destroy(vfrb_search_plan* p) {
     (void)hipFree(p->d_u8_off);
     (void)hipFree(p->d_u8_offsum);
     (void)hipFree(p->d_tmp_fb);
+    (void)hipFree(p->d_shift);
     delete p;
 }
 
@@ -1676,6 +1730,9 @@ vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* p, int32_t nrows, flo
         const vfrb_status st = build_tile_set(p, kStatsTile, 0, p->sts);
         if (st != VFRB_OK) return st;
     }
+    if (!p->d_shift) VFRB_HIP(hipMalloc(&p->d_shift, sizeof(float)));
+    hipLaunchKernelGGL(stats_shift_kernel, dim3(1), dim3(256), 0, vfrb::as_stream(stream), (const float*)p->d_win,
+                       p->ld, p->nch, nrows, p->d_shift);
     constexpr TileCfg tc = kStatsTile;
     const int64_t nwg = vfrb::ceil_div(nt, 64 * tc.sl) * p->sts.nblk;
     VFRB_REQUIRE(nwg < INT32_MAX, VFRB_ERR_UNSUPPORTED, "%lld dedispersion tiles", (long long)nwg);
@@ -1684,7 +1741,7 @@ vfrb_status vfrb_search_dedisperse_stats(vfrb_search_plan* p, int32_t nrows, flo
     hipLaunchKernelGGL((dedisp_stats_kernel<tc.sl, tc.dw, tc.w>), dim3((unsigned)nwg), dim3(64 * tc.w),
                        p->sts.lds_bytes, vfrb::as_stream(stream), (const float*)p->d_win, p->ld, p->nch, nrows,
                        nt, p->ndm, p->sts.d_roffs, p->sts.d_dmin, p->sts.d_chunks, p->sts.d_blk_chunks, out, sig,
-                       ld_out, p->sts.nblk);
+                       ld_out, p->sts.nblk, (const float*)p->d_shift);
     VFRB_LAUNCH_CHECK("dedisp_stats_kernel");
     return VFRB_OK;
 }
