@@ -1069,12 +1069,14 @@ def bench_serving_full(vfrb, args, ctx):
     pools = [vfrb.DeviceArray.from_host(host_payload[pol][:nbytes]) for pol in (0, 1)]
     nstreams, per_stream = 8, 3
     seg_s = seg_rows * tchan
+    # the tile the autotuner keeps on 20 s windows at this shape, for both servers
+    tile = vfrb.tile_names().index("G512_dm64_w16")
 
     def make(batched, storage):
         srv = vfrb.Server(n, nave, nch, p.nskip, p.fch1, p.chanbw, tchan, storage=storage,
                           acc=vfrb.ACC_F32C if storage != vfrb.STORE_F32 else vfrb.ACC_F64,
                           threshold=7.0, max_cand=1 << 16, max_streams=nstreams, max_segment_bytes=nbytes,
-                          arena_bytes=8 << 30, max_batch=4, batched=batched)
+                          arena_bytes=8 << 30, max_batch=4, batched=batched, tile=tile)
         ids = [srv.open_stream(dms) for _ in range(nstreams)]
         return srv, ids
 
@@ -1104,7 +1106,7 @@ def bench_serving_full(vfrb, args, ctx):
 
     modes = (("fp16_batched", True, vfrb.STORE_F16), ("fp32_plain", False, vfrb.STORE_F32))
     res = {"workload": f"{nstreams} streams x {per_stream} segments of {seg_s:.2f} s, 4096 ch x 2048 DM "
-                       "(0-1000), device-resident payload read in place",
+                       "(0-1000), device-resident payload read in place, tile G512_dm64_w16",
            "closed_loop": {}}
     for name, batched, storage in modes:
         srv, ids = make(batched, storage)
