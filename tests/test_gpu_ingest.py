@@ -116,6 +116,41 @@ def test_filterbank_from_frames_equals_deframed(gpu):
         fbp.execute_frames(raw, table, nslots, 0, nslots * 5000 // rowb + 1, a)
 
 
+@pytest.mark.parametrize("n", [2048, 8192, 16384])
+def test_filterbank_from_frames_slots_far_apart(gpu, n):
+    """Consecutive payload slots read from frame pairs more than 4 GB apart
+    in one buffer (and missing slots): the filterbank reads each slot where
+    the table puts it, equal to the filterbank of the gathered bytes."""
+    vfrb = gpu
+    rng = np.random.default_rng(n)
+    npairs, far = 460000, 459900          # 4.63 GB of frame pairs; pair `far` lies past 4 GiB
+    nslots = 96
+    raw = vfrb.DeviceArray(npairs * 2 * 5032, np.uint8)
+    for first in (0, far):
+        blk = rng.integers(0, 256, 64 * 2 * 5032, dtype=np.uint8)
+        vfrb.call("vfrb_memcpy_h2d", raw.offset(first * 2 * 5032), blk.ctypes.data_as(vfrb.C.c_void_p),
+                  blk.nbytes, None)
+    t = np.where(np.arange(nslots) % 2 == 0, np.arange(nslots) // 2, far + np.arange(nslots) // 2)
+    t[[5, 6, 40]] = -1
+    table = vfrb.DeviceArray.from_host(t.astype(np.int32))
+    nave = 2
+    fbp = vfrb.FilterbankPlan(n, nave, 1, n // 2)
+    pols = []
+    for pol in (0, 1):
+        o = vfrb.DeviceArray(nslots * 5000, np.uint8)
+        vfrb.Ingest.gather(raw, table, nslots, pol, o)
+        pols.append(o)
+    rowb = n * nave
+    for byte0 in (0, 16 * 195):
+        nrows = (nslots * 5000 - byte0) // rowb
+        print(f"[case] {n}-point filterbank, slots far apart, byte0 {byte0}, {nrows} rows", flush=True)
+        a = vfrb.DeviceArray((nrows, n // 2), np.float32)
+        b = vfrb.DeviceArray((nrows, n // 2), np.float32)
+        fbp.execute_frames(raw, table, nslots, byte0, nrows, a)
+        fbp.execute(pols[0], pols[1], nrows, b, pol_offset=byte0)
+        assert np.array_equal(a.download(), b.download())
+
+
 def test_ingest_failed_call_keeps_the_parse_state(gpu):
     """A batch that fails its capacity check leaves the carried parse state
     (expected frame) untouched: retrying it with a larger table, then the

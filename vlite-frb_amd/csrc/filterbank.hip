@@ -303,6 +303,8 @@ struct P2 {
     // formed once) where that fits the 256-VGPR budget without spilling:
     // 8192 points 3.84 -> 3.72 ms per 20 s segment, 2048 points -3 %.
     static constexpr bool HOIST = N == 1024 || N == 2048 || N == 8192;
+    // Separate |Z|^2 sums of the last pass's two butterflies (16 more VGPRs)
+    static constexpr bool SPLIT = N >= 1024;
     static constexpr int cmax(int a, int b) { return a > b ? a : b; }
     static constexpr int LDS_WORDS =                      // per component
         cmax(PadA::words(TOT), cmax(MID1 > 1 ? PadB::words(TOT) : 0, PadC::words(TOT)));
@@ -365,8 +367,9 @@ __device__ __forceinline__ float norm2(cf a) { return a.x * a.x + a.y * a.y; }
 // SIMD: hold the register allocator to that budget.
 // Raw VDIF frames (FRAMED): pol0 points at frame pairs (thread 0 then 1,
 // 5032 bytes each), table[s] is the pair feeding payload slot s (5000 bytes
-// per pol) or -1 for a zero-filled slot, and byte0 is the payload-stream
-// offset of row 0.  Otherwise pol0/pol1 are the de-framed payload streams.
+// per pol) or -1 for a zero-filled slot, byte0 is the payload-stream offset
+// of row 0, and pol1 points at one frame pair of zero bytes.  Otherwise
+// pol0/pol1 are the de-framed payload streams.
 constexpr int kFrameBytes = 5032, kHeaderBytes = 32, kPayloadBytes = 5000;
 
 template <int N, bool FRAMED>
@@ -385,20 +388,57 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
     const int tid = threadIdx.x;
     const int64_t row = blockIdx.x;
     if (row >= nrows) return;
-    const int slot = tid / TPF;
+    // the FFT slot of the workgroup; wave-uniform where an FFT spans whole waves
+    const int slot = TPF >= 64 ? __builtin_amdgcn_readfirstlane(tid / TPF) : tid / TPF;
     const int t = tid % TPF;
     const int fbase = slot * N;
+    const uint8_t* zero = pol1;  // FRAMED: a zeroed frame pair, read for missing slots
 
     // Last-pass butterflies of this thread and their twiddles (frame invariant).
     const int ja = t == 0 ? 0 : t;
     const int jb = t == 0 ? N / 32 : NS - t;
 
-    float acc[16];
+    // Where registers allow (C::SPLIT), |Z|^2 of the two last-pass butterflies
+    // is summed over the frames separately (sa, sb: two FMAs per bin and
+    // frame) and the conjugate bins are paired once after the frame loop;
+    // otherwise acc sums the pairs every frame.
+    float acc[16], sa[C::SPLIT ? 16 : 1], sb[C::SPLIT ? 16 : 1];
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+#pragma unroll
+    for (int i = 0; i < (C::SPLIT ? 16 : 1); i++) sa[i] = sb[i] = 0.0f;
 
     // prefetch registers: 8 dwords per pol = samples 4t..4t+3 + r*N/8
     uint32_t x0[8], x1[8];
+    // FRAMED, TPF >= 64: payload slot fs and byte fw in it where the wave's
+    // next frame starts (wave-uniform; one FP64 division per row, then steps)
+    int64_t fs = 0;
+    int fw = 0;
+    // and the table entries of the slots that frame spans, loaded one frame
+    // ahead so that no wave waits for them
+    constexpr int FMAXS = (N + kPayloadBytes - 1) / kPayloadBytes + 1;
+    int tk[FMAXS];
+    auto load_tk = [&]() {
+        const int ns = (fw + N - 1) / kPayloadBytes;
+#pragma unroll
+        for (int j = 0; j < FMAXS; j++) tk[j] = j <= ns ? table[fs + j] : -1;
+    };
+    if constexpr (FRAMED && TPF >= 64) {
+        const int64_t of = byte0 + ((int64_t)row * nave + slot) * N;
+        int64_t s = (int64_t)((double)of * (1.0 / kPayloadBytes));
+        int w = (int)(of - s * kPayloadBytes);
+        if (w < 0) {
+            s--;
+            w += kPayloadBytes;
+        } else if (w >= kPayloadBytes) {
+            s++;
+            w -= kPayloadBytes;
+        }
+        fs = ((int64_t)__builtin_amdgcn_readfirstlane((int)(s >> 32)) << 32) |
+             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)s);
+        fw = __builtin_amdgcn_readfirstlane(w);
+        if (slot < nave) load_tk();
+    }
     auto fetch = [&](int it) {
         const int frame = it + slot;
         if (frame >= nave) {
@@ -410,44 +450,75 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             return;
         }
         if constexpr (FRAMED && TPF >= 64) {
-            // One FFT frame per wave: the wave resolves the few payload slots
-            // its N bytes span (scalar loads of the table), every lane picks
-            // the slot of each of its dwords by comparison.
+            // The FFT frame's N bytes span at most MAXS payload slots, from
+            // slot fs on (byte fw of it).  Their bases are wave-uniform,
+            // computed once per frame (a missing slot reads the zero pair
+            // `zero`); each lane then picks, per dword, one of them by
+            // comparing its byte position with the slot edges.
             constexpr int MAXS = (N + kPayloadBytes - 1) / kPayloadBytes + 1;
-            const int64_t of = byte0 + ((int64_t)row * nave + frame) * N;
-            int64_t s0 = (int64_t)((double)of * (1.0 / kPayloadBytes));
-            int w0 = (int)(of - s0 * kPayloadBytes);
-            if (w0 < 0) {
-                s0--;
-                w0 += kPayloadBytes;
-            } else if (w0 >= kPayloadBytes) {
-                s0++;
-                w0 -= kPayloadBytes;
-            }
-            s0 = ((int64_t)__builtin_amdgcn_readfirstlane((int)(s0 >> 32)) << 32) |
-                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)s0);
-            w0 = __builtin_amdgcn_readfirstlane(w0);
-            const int nspan = (w0 + N - 1) / kPayloadBytes;  // last slot index used, < MAXS
-            int ks[MAXS];
+            static_assert(MAXS == FMAXS, "one slot count");
+            const int w0 = fw;
+            int kk[MAXS];
 #pragma unroll
-            for (int j = 0; j < MAXS; j++) ks[j] = j <= nspan ? table[s0 + j] : -1;
+            for (int j = 0; j < MAXS; j++) kk[j] = tk[j];
+            fw += C::FPB * N;  // the wave's next frame
+            const int q = fw / kPayloadBytes;
+            fs += q;
+            fw -= q * kPayloadBytes;
+            if (frame + C::FPB < nave) load_tk();
+            // byte p of the frame's payload run (p counted from slot 0's first
+            // byte) lies in slot j = p / 5000, at base[j] + p (slots past the
+            // frame's last have the entry -1)
+            const uint8_t* base[MAXS];
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const int pos = w0 + 4 * t + r * EIGHTH;
-                int j = 0;
+            for (int j = 0; j < MAXS; j++)
+                base[j] = (kk[j] >= 0 ? pol0 + (int64_t)kk[j] * (2 * kFrameBytes) : zero) + kHeaderBytes -
+                          j * kPayloadBytes;
+            const uint32_t p0 = (uint32_t)w0 + 4u * (uint32_t)t;
+            // Where the bases lie within 4 GB of the lowest, a lane's address
+            // is that base (scalar) plus a 32-bit offset.
+            const uint8_t* lo = base[0];
 #pragma unroll
-                for (int m = 1; m < MAXS; m++) j += pos >= m * kPayloadBytes ? 1 : 0;
-                int k = ks[0];
+            for (int j = 1; j < MAXS; j++) lo = base[j] < lo ? base[j] : lo;
+            uint64_t span = 0;
 #pragma unroll
-                for (int m = 1; m < MAXS; m++) k = j == m ? ks[m] : k;
-                uint32_t a0 = 0u, a1 = 0u;  // zero-filled payload of a missing frame
-                if (k >= 0) {
-                    const uint8_t* f = pol0 + (int64_t)k * (2 * kFrameBytes) + kHeaderBytes + (pos - j * kPayloadBytes);
-                    a0 = *reinterpret_cast<const uint32_t*>(f);
-                    a1 = *reinterpret_cast<const uint32_t*>(f + kFrameBytes);
+            for (int j = 0; j < MAXS; j++) span = base[j] - lo > span ? (uint64_t)(base[j] - lo) : span;
+            if (span < (1ull << 32) - (uint64_t)(MAXS + 2) * kPayloadBytes - 2 * kFrameBytes) {
+                uint32_t d[MAXS];
+#pragma unroll
+                for (int j = 0; j < MAXS; j++) d[j] = (uint32_t)(base[j] - lo);
+                // scalar bases of the two pols, the first four dwords' and the
+                // last four's (the rest of r*N/8 is an instruction offset); the
+                // empty asm keeps them whole scalar values
+                using gbyte = const __attribute__((address_space(1))) uint8_t;
+                using gword = const __attribute__((address_space(1))) uint32_t;
+                gbyte* sb[4] = {(gbyte*)lo, (gbyte*)(lo + 4 * EIGHTH), (gbyte*)(lo + kFrameBytes),
+                                (gbyte*)(lo + 4 * EIGHTH + kFrameBytes)};
+#pragma unroll
+                for (int k = 0; k < 4; k++) asm volatile("" : "+s"(sb[k]));
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    uint32_t o = d[0];
+#pragma unroll
+                    for (int m = 1; m < MAXS; m++) {
+                        const int th = m * kPayloadBytes - r * EIGHTH;  // p0 >= th: slot m or later
+                        if (th <= 0 || p0 >= (uint32_t)th) o = d[m];
+                    }
+                    const size_t v = p0 + o;
+                    const int ro = (r & 3) * EIGHTH;
+                    x0[r] = *reinterpret_cast<gword*>(sb[r >> 2] + v + ro);
+                    x1[r] = *reinterpret_cast<gword*>(sb[2 + (r >> 2)] + v + ro);
                 }
-                x0[r] = a0;
-                x1[r] = a1;
+            } else {
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    const uint32_t p = p0 + (uint32_t)(r * EIGHTH);
+                    const uint8_t* b = base[0];
+#pragma unroll
+                    for (int m = 1; m < MAXS; m++) b = p >= (uint32_t)(m * kPayloadBytes) ? base[m] : b;
+                    x0[r] = *reinterpret_cast<const uint32_t*>(b + p);
+                    x1[r] = *reinterpret_cast<const uint32_t*>(b + p + kFrameBytes);
+                }
             }
         } else if constexpr (FRAMED) {
             // payload offset -> (slot, byte): one FP64 division, then steps of
@@ -565,7 +636,10 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             apply_twiddle_set<16>(a, tw_a);
             dft<16>(a);
 #pragma unroll
-            for (int r = 0; r < 16; r++) na[r] = norm2(a[r]);
+            for (int r = 0; r < 16; r++) {
+                if constexpr (C::SPLIT) sa[r] = fmaf(a[r].x, a[r].x, fmaf(a[r].y, a[r].y, sa[r]));
+                else na[r] = norm2(a[r]);
+            }
         }
         {
             cf b[16];
@@ -578,8 +652,12 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             apply_twiddle_set<16>(b, tw_b);
             dft<16>(b);
 #pragma unroll
-            for (int r = 0; r < 16; r++) nb[r] = norm2(b[r]);
+            for (int r = 0; r < 16; r++) {
+                if constexpr (C::SPLIT) sb[r] = fmaf(b[r].x, b[r].x, fmaf(b[r].y, b[r].y, sb[r]));
+                else nb[r] = norm2(b[r]);
+            }
         }
+        if constexpr (C::SPLIT) continue;
         // ---- detect: |Z[k]|^2 + |Z[N-k]|^2
         if (t != 0) {
 #pragma unroll
@@ -592,6 +670,21 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             for (int r = 0; r < 8; r++) {
                 acc[r] += nb[r] + nb[15 - r];          // k = N/32 + r*NS
                 acc[8 + r] += na[r + 1] + na[15 - r];  // k = (r+1)*NS
+            }
+        }
+    }
+    if constexpr (C::SPLIT) {  // pair the conjugate bins: |Z[k]|^2 + |Z[N-k]|^2
+        if (t != 0) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                acc[r] = sa[r] + sb[15 - r];      // k = ja + r*NS
+                acc[8 + r] = sb[r] + sa[15 - r];  // k = jb + r*NS
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                acc[r] = sb[r] + sb[15 - r];          // k = N/32 + r*NS
+                acc[8 + r] = sa[r + 1] + sa[15 - r];  // k = (r+1)*NS
             }
         }
     }
@@ -877,6 +970,7 @@ struct vfrb_fb_plan {
     bool pow2 = false;
     GenPlan gen{};
     float2* d_tw = nullptr;
+    uint8_t* d_zero = nullptr;  // one frame pair of zeros: the framed kernel's missing slots
     size_t lds_bytes = 0;
     int block = 256;
     const void* kernel = nullptr;
@@ -970,12 +1064,15 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
     }
     hipError_t e = hipMalloc(&p->d_tw, sizeof(float2) * N);
     if (e == hipSuccess) e = hipMemcpy(p->d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice);
+    if (e == hipSuccess && p->kernel_framed) e = hipMalloc(&p->d_zero, 2 * kFrameBytes);
+    if (e == hipSuccess && p->kernel_framed) e = hipMemset(p->d_zero, 0, 2 * kFrameBytes);
     if (e == hipSuccess && p->lds_bytes > 65536)
         e = hipFuncSetAttribute(p->kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes);
     if (e == hipSuccess && p->kernel_framed && p->lds_bytes > 65536)
         e = hipFuncSetAttribute(p->kernel_framed, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds_bytes);
     if (e != hipSuccess) {
         if (p->d_tw) (void)hipFree(p->d_tw);
+        if (p->d_zero) (void)hipFree(p->d_zero);
         delete p;
         return vfrb::hip_fail(e, "vfrb_fb_plan_create", __FILE__, __LINE__);
     }
@@ -986,6 +1083,7 @@ vfrb_status vfrb_fb_plan_create(vfrb_fb_plan** plan, int32_t twonchan, int32_t n
 vfrb_status vfrb_fb_plan_destroy(vfrb_fb_plan* p) {
     if (!p) return VFRB_OK;
     if (p->d_tw) VFRB_HIP(hipFree(p->d_tw));
+    if (p->d_zero) VFRB_HIP(hipFree(p->d_zero));
     delete p;
     return VFRB_OK;
 }
@@ -1037,8 +1135,8 @@ vfrb_status vfrb_fb_execute_frames(vfrb_fb_plan* p, const uint8_t* frames, const
                  VFRB_ERR_INVALID, "rows reach past the %lld table slots", (long long)nslots);
     if (nrows == 0) return VFRB_OK;
     const float scale = (float)(0.5 / ((double)(p->N / 2 + 1) * (double)p->nave));
-    const uint8_t* none = nullptr;
-    void* args[] = {(void*)&frames, (void*)&none, (void*)&p->d_tw, (void*)&fb,  (void*)&nrows,
+    const uint8_t* zero = p->d_zero;
+    void* args[] = {(void*)&frames, (void*)&zero, (void*)&p->d_tw, (void*)&fb,  (void*)&nrows,
                     (void*)&p->nave, (void*)&p->nskip, (void*)&p->nchanout, (void*)&ring_rows,
                     (void*)&row0, (void*)&scale, (void*)&table, (void*)&byte0};
     VFRB_HIP(hipLaunchKernel(p->kernel_framed, dim3((unsigned)nrows), dim3(p->block), args, p->lds_bytes,
