@@ -371,6 +371,7 @@ __device__ __forceinline__ float norm2(cf a) { return a.x * a.x + a.y * a.y; }
 // of row 0, and pol1 points at one frame pair of zero bytes.  Otherwise
 // pol0/pol1 are the de-framed payload streams.
 constexpr int kFrameBytes = 5032, kHeaderBytes = 32, kPayloadBytes = 5000;
+constexpr int kFbBufferFlags = 0x00020000;  // word 3 of a gfx9 raw buffer resource
 
 template <int N, bool FRAMED>
 __global__ void __launch_bounds__(P2<N>::BLOCK) __attribute__((amdgpu_waves_per_eu(2, 2)))
@@ -451,51 +452,42 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
         }
         if constexpr (FRAMED && TPF >= 64) {
             // The FFT frame's N bytes span at most MAXS payload slots, from
-            // slot fs on (byte fw of it).  Their bases are wave-uniform,
-            // computed once per frame (a missing slot reads the zero pair
-            // `zero`); each lane then picks, per dword, one of them by
-            // comparing its byte position with the slot edges.
+            // slot fs on (byte fw of it), whose frame pairs tk[] holds.  The
+            // slots' addresses are wave-uniform, formed once per frame; each
+            // lane then picks, per dword, one of them by comparing its byte
+            // position with the slot edges.
             constexpr int MAXS = (N + kPayloadBytes - 1) / kPayloadBytes + 1;
             static_assert(MAXS == FMAXS, "one slot count");
             const int w0 = fw;
-            int kk[MAXS];
-#pragma unroll
-            for (int j = 0; j < MAXS; j++) kk[j] = tk[j];
-            fw += C::FPB * N;  // the wave's next frame
-            const int q = fw / kPayloadBytes;
-            fs += q;
-            fw -= q * kPayloadBytes;
-            if (frame + C::FPB < nave) load_tk();
-            // byte p of the frame's payload run (p counted from slot 0's first
-            // byte) lies in slot j = p / 5000, at base[j] + p (slots past the
-            // frame's last have the entry -1)
-            const uint8_t* base[MAXS];
-#pragma unroll
-            for (int j = 0; j < MAXS; j++)
-                base[j] = (kk[j] >= 0 ? pol0 + (int64_t)kk[j] * (2 * kFrameBytes) : zero) + kHeaderBytes -
-                          j * kPayloadBytes;
+            const int ns = (w0 + N - 1) / kPayloadBytes;  // last slot of the frame, < MAXS
             const uint32_t p0 = (uint32_t)w0 + 4u * (uint32_t)t;
-            // Where the bases lie within 4 GB of the lowest, a lane's address
-            // is that base (scalar) plus a 32-bit offset.
-            const uint8_t* lo = base[0];
+            // Lowest and highest frame pair of the frame's slots: where no
+            // slot is missing and they lie within 400000 pairs (4 GB), a
+            // lane's address is one scalar base plus a 32-bit offset.
+            int kmin = tk[0], kmax = tk[0];
 #pragma unroll
-            for (int j = 1; j < MAXS; j++) lo = base[j] < lo ? base[j] : lo;
-            uint64_t span = 0;
-#pragma unroll
-            for (int j = 0; j < MAXS; j++) span = base[j] - lo > span ? (uint64_t)(base[j] - lo) : span;
-            if (span < (1ull << 32) - (uint64_t)(MAXS + 2) * kPayloadBytes - 2 * kFrameBytes) {
+            for (int j = 1; j < MAXS; j++)
+                if (j <= ns) {
+                    kmin = min(kmin, tk[j]);
+                    kmax = max(kmax, tk[j]);
+                }
+            if (kmin >= 0 && kmax - kmin < 400000) {
+                // slot j's payload byte p (p counted from the frame's slot 0)
+                // is at lo + d[j] + p; d[j] < 2^32 - 2^22
+                const uint8_t* lo = pol0 + (int64_t)kmin * (2 * kFrameBytes) + kHeaderBytes -
+                                    (MAXS - 1) * kPayloadBytes;
                 uint32_t d[MAXS];
 #pragma unroll
-                for (int j = 0; j < MAXS; j++) d[j] = (uint32_t)(base[j] - lo);
-                // scalar bases of the two pols, the first four dwords' and the
-                // last four's (the rest of r*N/8 is an instruction offset); the
-                // empty asm keeps them whole scalar values
-                using gbyte = const __attribute__((address_space(1))) uint8_t;
-                using gword = const __attribute__((address_space(1))) uint32_t;
-                gbyte* sb[4] = {(gbyte*)lo, (gbyte*)(lo + 4 * EIGHTH), (gbyte*)(lo + kFrameBytes),
-                                (gbyte*)(lo + 4 * EIGHTH + kFrameBytes)};
-#pragma unroll
-                for (int k = 0; k < 4; k++) asm volatile("" : "+s"(sb[k]));
+                for (int j = 0; j < MAXS; j++)
+                    d[j] = j <= ns ? (uint32_t)(tk[j] - kmin) * (uint32_t)(2 * kFrameBytes) +
+                                         (uint32_t)((MAXS - 1 - j) * kPayloadBytes)
+                                   : 0u;
+                // one raw buffer per pol based at lo: the lane's 32-bit offset,
+                // r*N/8 as scalar and instruction offsets
+                const __amdgpu_buffer_rsrc_t rs0 =
+                    __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(lo), 0, -1, kFbBufferFlags);
+                const __amdgpu_buffer_rsrc_t rs1 = __builtin_amdgcn_make_buffer_rsrc(
+                    const_cast<uint8_t*>(lo + kFrameBytes), 0, -1, kFbBufferFlags);
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     uint32_t o = d[0];
@@ -504,12 +496,19 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                         const int th = m * kPayloadBytes - r * EIGHTH;  // p0 >= th: slot m or later
                         if (th <= 0 || p0 >= (uint32_t)th) o = d[m];
                     }
-                    const size_t v = p0 + o;
-                    const int ro = (r & 3) * EIGHTH;
-                    x0[r] = *reinterpret_cast<gword*>(sb[r >> 2] + v + ro);
-                    x1[r] = *reinterpret_cast<gword*>(sb[2 + (r >> 2)] + v + ro);
+                    const int v = (int)(p0 + o + (uint32_t)((r & 3) * EIGHTH));
+                    const int so = (r >> 2) * 4 * EIGHTH;
+                    x0[r] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs0, v, so, 0);
+                    x1[r] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rs1, v, so, 0);
                 }
             } else {
+                // missing slots (the zero pair) or pairs far apart: 64-bit
+                // addresses, slot j's byte p at base[j] + p
+                const uint8_t* base[MAXS];
+#pragma unroll
+                for (int j = 0; j < MAXS; j++)
+                    base[j] = (tk[j] >= 0 ? pol0 + (int64_t)tk[j] * (2 * kFrameBytes) : zero) + kHeaderBytes -
+                              j * kPayloadBytes;
 #pragma unroll
                 for (int r = 0; r < 8; r++) {
                     const uint32_t p = p0 + (uint32_t)(r * EIGHTH);
@@ -520,6 +519,13 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                     x1[r] = *reinterpret_cast<const uint32_t*>(b + p + kFrameBytes);
                 }
             }
+            // the wave's next frame, and its table entries (read now, used in
+            // the next call)
+            fw += C::FPB * N;
+            const int q = fw / kPayloadBytes;
+            fs += q;
+            fw -= q * kPayloadBytes;
+            if (frame + C::FPB < nave) load_tk();
         } else if constexpr (FRAMED) {
             // payload offset -> (slot, byte): one FP64 division, then steps of
             // N/8 < 5000 bytes carry into the next slot at most once
