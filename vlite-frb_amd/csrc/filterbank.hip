@@ -694,33 +694,36 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
             }
         }
     }
-    // ---- sum the FFT slots of this row, then write the selected channels
+    // ---- sum the FFT slots of this row, then write the selected channels:
+    // accumulator i of butterfly thread u holds bin k(i, u)
+    int64_t orow = row0 + row;
+    if (ring_rows > 0) orow %= ring_rows;
+    float* dst = fb + orow * (int64_t)nchanout;
+    auto bin = [&](int i, int u) {
+        const int r = i & 7;
+        if (u != 0) return (i < 8 ? u : NS - u) + r * NS;
+        return i < 8 ? N / 32 + r * NS : (r + 1) * NS;
+    };
     if constexpr (C::FPB > 1) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 16; i++) re[i * C::BLOCK + tid] = acc[i];
         __syncthreads();
-        if (slot == 0) {
+        // one (accumulator, butterfly thread) pair per thread, summed over the
+        // FFT slots in slot order
+        for (int o = tid; o < 16 * TPF; o += C::BLOCK) {
+            const int i = o / TPF, u = o % TPF;
+            float s = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                float s = acc[i];
-#pragma unroll
-                for (int f = 1; f < C::FPB; f++) s += re[i * C::BLOCK + f * TPF + t];
-                acc[i] = s;
-            }
+            for (int f = 0; f < C::FPB; f++) s += re[i * C::BLOCK + f * TPF + u];
+            const int c = bin(i, u) - nskip;
+            if (c >= 0 && c < nchanout) dst[c] = s * scale;
         }
+        return;
     }
-    if (slot != 0) return;
-    int64_t orow = row0 + row;
-    if (ring_rows > 0) orow %= ring_rows;
-    float* dst = fb + orow * (int64_t)nchanout;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int r = i & 7;
-        int k;
-        if (t != 0) k = (i < 8 ? ja : jb) + r * NS;
-        else k = i < 8 ? N / 32 + r * NS : (r + 1) * NS;
-        const int c = k - nskip;
+        const int c = bin(i, t) - nskip;
         if (c >= 0 && c < nchanout) dst[c] = acc[i] * scale;
     }
 }
