@@ -157,7 +157,8 @@ vfrb_status vfrb_fb_plan_destroy(vfrb_fb_plan* plan);
  * samples each (device).  Output row r goes to
  *   fb[((row0 + r) mod ring_rows) * nchanout + c],  c < nchanout,
  * with fb[.][c] = (1/nave) * sum_frames (|X0[nskip+c]|^2+|X1[nskip+c]|^2)/nfreq.
- * ring_rows = 0 means a linear buffer (no wrap).
+ * ring_rows = 0 means a linear buffer (no wrap).  Power-of-two lengths read
+ * the pols as 32-bit words: pol0 and pol1 must be 4-byte aligned.
  */
 vfrb_status vfrb_fb_execute(vfrb_fb_plan* plan, const uint8_t* pol0,
                             const uint8_t* pol1, int64_t nrows, float* fb,

@@ -298,9 +298,9 @@ def test_server_device_input_in_place_equals_copy(gpu):
     p1 = vfrb.synth_payload(spec, 1, 0, nblk)[:total]
     cuts = np.concatenate([[0], np.cumsum(nbytes)])
     res = []
-    # host input; device input in place (rows 4-byte aligned); device input
-    # one byte off alignment (every segment through the arena copy)
-    for device, pad in ((False, 0), (True, 0), (True, 1)):
+    # host input; device input in place (rows 16- and 4-byte aligned); device
+    # input one byte off alignment (every segment through the arena copy)
+    for device, pad in ((False, 0), (True, 0), (True, 4), (True, 1)):
         print(f"[case] server device input in place: device={device} pad={pad}", flush=True)
         srv = _server(vfrb, p, vfrb.STORE_F16, batched=True)
         sid = srv.open_stream(DMS)

@@ -1106,8 +1106,8 @@ vfrb_status vfrb_fb_execute(vfrb_fb_plan* p, const uint8_t* pol0, const uint8_t*
     VFRB_REQUIRE(nrows <= INT32_MAX, VFRB_ERR_INVALID, "nrows too large for one launch");
     if (nrows == 0) return VFRB_OK;
     if (p->pow2) {
-        VFRB_REQUIRE(((uintptr_t)pol0 % 16) == 0 && ((uintptr_t)pol1 % 16) == 0, VFRB_ERR_INVALID,
-                     "pol buffers must be 16-byte aligned");
+        VFRB_REQUIRE(((uintptr_t)pol0 % 4) == 0 && ((uintptr_t)pol1 % 4) == 0, VFRB_ERR_INVALID,
+                     "pol buffers must be 4-byte aligned");
     }
     const float scale = (float)(0.5 / ((double)(p->N / 2 + 1) * (double)p->nave));
     hipStream_t s = vfrb::as_stream(stream);
