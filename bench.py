@@ -1006,7 +1006,8 @@ def bench_serving(vfrb, args):
         wall = time.perf_counter() - t0
         got += srv.poll(wait_ms=0, max_results=4096)
         lat = np.array([r["done_s"] - r["arrival_s"] for r in got])
-        seglen = np.array([ev[r["tag"]][3] / rate for r in got])
+        # throughput counts the segments that completed without error
+        seglen = np.array([ev[r["tag"]][3] / rate if r["status"] == 0 else 0.0 for r in got])
         bs = np.array([r["batch_size"] for r in got])
         st = srv.stats()
         return {"segments": len(got), "data_s": float(seglen.sum()), "wall_s": wall,
@@ -1033,7 +1034,8 @@ def bench_serving(vfrb, args):
                 res.setdefault(inp, {}).setdefault(f"load_{load:g}x", {})[name] = r
                 log(f"serving {inp} load {load}x {name}: {r['segments']} segs, p50 "
                     f"{r['p50_latency_ms']:.1f} ms, p99 {r['p99_latency_ms']:.1f} ms, thr "
-                    f"{r['throughput_data_s_per_s']:.1f} data-s/s, mean batch {r['mean_batch']:.2f}")
+                    f"{r['throughput_data_s_per_s']:.1f} data-s/s, mean batch {r['mean_batch']:.2f}"
+                    + (f", {r['errors']} SEGMENTS FAILED" if r["errors"] else ""))
         # capacity: a 2x trace submitted at once
         ev = trace(2.0)
         for name, batched, storage in modes:
@@ -1042,7 +1044,8 @@ def bench_serving(vfrb, args):
             srv.close()
             res[inp].setdefault("closed_loop", {})[name] = r
             log(f"serving {inp} closed loop {name}: {r['throughput_data_s_per_s']:.1f} data-s/s, "
-                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}"
+                + (f", {r['errors']} SEGMENTS FAILED" if r["errors"] else ""))
         c = res[inp]["closed_loop"]
         res[inp]["capacity_ratio_fp16_batched_vs_fp32_plain"] = (
             c["fp16_batched"]["throughput_data_s_per_s"] / c["fp32_plain"]["throughput_data_s_per_s"])
@@ -1096,8 +1099,9 @@ def bench_serving_full(vfrb, args, ctx):
         lat = np.array([r["done_s"] - r["arrival_s"] for r in got])
         bs = np.array([r["batch_size"] for r in got])
         st = srv.stats()
-        return {"segments": len(got), "data_s": nseg * seg_s, "wall_s": wall,
-                "throughput_data_s_per_s": nseg * seg_s / wall,
+        nok = sum(r["status"] == 0 for r in got)  # throughput counts error-free segments
+        return {"segments": len(got), "data_s": nok * seg_s, "wall_s": wall,
+                "throughput_data_s_per_s": nok * seg_s / wall,
                 "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
                 "p99_latency_over_segment": float(np.percentile(lat, 99) / seg_s),
@@ -1117,7 +1121,8 @@ def bench_serving_full(vfrb, args, ctx):
         srv.close()
         res["closed_loop"][name] = r
         log(f"serving 4096x2048 closed loop {name}: {r['throughput_data_s_per_s']:.1f} data-s/s, "
-            f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+            f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}"
+            + (f", {r['errors']} SEGMENTS FAILED" if r["errors"] else ""))
     cap = res["closed_loop"]["fp32_plain"]["throughput_data_s_per_s"]
     res["capacity_ratio_fp16_batched_vs_fp32_plain"] = (
         res["closed_loop"]["fp16_batched"]["throughput_data_s_per_s"] / cap)
@@ -1132,7 +1137,8 @@ def bench_serving_full(vfrb, args, ctx):
             r["offered_data_s_per_s"] = frac * cap
             res.setdefault(f"poisson_{int(frac * 100)}pct_of_fp32_capacity", {})[name] = r
             log(f"serving 4096x2048 at {frac:.0%} of FP32 capacity {name}: p50 {r['p50_latency_ms']:.1f} ms, "
-                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}")
+                f"p99 {r['p99_latency_ms']:.1f} ms, mean batch {r['mean_batch']:.2f}"
+                + (f", {r['errors']} SEGMENTS FAILED" if r["errors"] else ""))
     if "headline_fp32_rtf" in ctx:
         res["ratio_fp16_batched_server_vs_headline_fp32_pipeline"] = (
             res["closed_loop"]["fp16_batched"]["throughput_data_s_per_s"] / ctx["headline_fp32_rtf"])
