@@ -239,8 +239,10 @@ constexpr size_t kDataBudget = 60 * 1024;
 constexpr int kBufferFlags = 0x00020000;  // word 3 of a gfx9 raw buffer resource
 
 // Group tiles: presum rows per chunk at most, and the record of each row
-// (8 ints: channel of its group within the chunk, the group's start row, the
-// three relative delays of the group's other channels, its channel count).
+// (8 ints: for each of the group's 4 channels the byte offset, from the
+// chunk's first channel row at time 0, of the dword that holds the channel's
+// first element; the elements' phases in those dwords, one per byte; the
+// row's LDS byte offset; the group's first channel in the chunk).
 constexpr int kRowMax = 48;
 constexpr int kRowInts = 8;
 
@@ -757,6 +759,27 @@ struct GroupStageItems {
     static constexpr int value = need < cap ? need : cap;
 };
 
+// f16 group presum of 4 samples: after the phase shift, channel c's dwords
+// hold elements (0,1) and (2,3).  v_perm pairs element e of two channels in
+// one half2, and v_dot2c_f32_f16 with (1,1) adds both into an f32 sum:
+// convert and add in one instruction per pair.
+__device__ __forceinline__ float4 group_sum_f16(const uint32_t (&al)[kGroupCh][2]) {
+    constexpr h2_t kOnes = {(_Float16)1.0f, (_Float16)1.0f};
+    float v[4];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int hsel = 0; hsel < 2; hsel++) {
+            const uint32_t sel = hsel ? 0x07060302u : 0x05040100u;
+            const uint32_t q01 = __builtin_amdgcn_perm(al[1][i], al[0][i], sel);
+            const uint32_t q23 = __builtin_amdgcn_perm(al[3][i], al[2][i], sel);
+            const float s01 = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q01), kOnes, 0.0f, false);
+            v[2 * i + hsel] = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q23), kOnes, s01, false);
+        }
+    }
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+
 template <typename TS, int SL, int DW, int W>
 __device__ __forceinline__ void
 dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
@@ -783,7 +806,6 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
     const int t0 = tx * T;
     const int dm0 = b * DMB + wave * DW;
     const int2 cr = uniform_load(blk_chunks + b);
-    const uint32_t ldw = (uint32_t)(ld / EPD);
 
     float acc[DW][SL];
 #pragma unroll
@@ -819,9 +841,11 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<TS*>(base), 0, (int)(left < 0x7fffffff ? left : 0x7fffffff), kBufferFlags);
         const float rng = 1.0f / (float)ng;
+        const uint32_t toff = 4u * (uint32_t)(t0 / EPD);  // t0 is a multiple of EPD
         // Item j = (row r, 4-sample group g): the row's four channels' elements
         // S + d_k + 4g .. +3 (buffer loads; zeros past the window or for a
         // channel the last group lacks) summed into presum samples 4g..4g+3.
+        // The row record holds each channel's byte offset and phase.
         auto stage = [&](int j0) {
             uint32_t vo[NI][kGroupCh];
             int ph[NI][kGroupCh], wo[NI];
@@ -836,23 +860,22 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
                 g -= fix * ng;
                 const bool ok = j < items;
                 const int* rec = rcur + (ok ? r : 0) * kRowInts;
-                const int crel = rec[0], nchg = rec[5];
-                const uint32_t e0 = (uint32_t)(t0 + rec[1] + 4 * g);
+                const int4 rbase = *reinterpret_cast<const int4*>(rec);
+                const int4 rmeta = *reinterpret_cast<const int4*>(rec + 4);
+                const int bases[kGroupCh] = {rbase.x, rbase.y, rbase.z, rbase.w};
+                const uint32_t lo = toff + (uint32_t)(16 * g / EPD);
 #pragma unroll
                 for (int c = 0; c < kGroupCh; c++) {
-                    const uint32_t e = e0 + (uint32_t)(c == 0 ? 0 : rec[1 + c]);
-                    vo[m][c] = ok && c < nchg ? 4u * ((uint32_t)(crel + c) * ldw + e / EPD) : 0x80000000u;
-                    ph[m][c] = (int)(e % EPD);
+                    vo[m][c] = ok ? (uint32_t)bases[c] + lo : 0x80000000u;
+                    ph[m][c] = (rmeta.x >> (8 * c)) & 3;
                     sc[m][c] = 1.0f;
-                    if constexpr (ESZ == 1) sc[m][c] = c < nchg ? chan_scale[ch.c0 + crel + c] : 0.0f;
+                    if constexpr (ESZ == 1) sc[m][c] = bases[c] >= 0 ? chan_scale[ch.c0 + rmeta.z + c] : 0.0f;
                 }
-                wo[m] = ok ? (r * ch.stride + 4 * g) * 4 : -1;
+                wo[m] = ok ? rmeta.y + 16 * g : -1;
             }
             if constexpr (ESZ == 2) {
-                // f16: after the phase shift, channel c's dwords hold elements
-                // (0,1) and (2,3).  v_perm pairs element e of two channels in
-                // one half2, and v_dot2c_f32_f16 with (1,1) adds both into an
-                // f32 sum: convert and add in one instruction per pair.
+                // f16: three raw dwords per channel, shifted by the phase and
+                // summed by group_sum_f16
                 uint32_t raw[NI][kGroupCh][3];
 #pragma unroll
                 for (int m = 0; m < NI; m++)
@@ -861,7 +884,6 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
 #pragma unroll
                         for (int i = 0; i < 3; i++)
                             raw[m][c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[m][c] + 4 * i, 0, 0);
-                constexpr h2_t kOnes = {(_Float16)1.0f, (_Float16)1.0f};
 #pragma unroll
                 for (int m = 0; m < NI; m++) {
                     if (wo[m] < 0) continue;
@@ -871,21 +893,7 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
 #pragma unroll
                         for (int i = 0; i < 2; i++)
                             al[c][i] = __builtin_amdgcn_alignbyte(raw[m][c][i + 1], raw[m][c][i], 2 * ph[m][c]);
-                    float v[4];
-#pragma unroll
-                    for (int i = 0; i < 2; i++) {
-#pragma unroll
-                        for (int hsel = 0; hsel < 2; hsel++) {
-                            const uint32_t sel = hsel ? 0x07060302u : 0x05040100u;
-                            const uint32_t q01 = __builtin_amdgcn_perm(al[1][i], al[0][i], sel);
-                            const uint32_t q23 = __builtin_amdgcn_perm(al[3][i], al[2][i], sel);
-                            const float s01 = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q01), kOnes,
-                                                                     0.0f, false);
-                            v[2 * i + hsel] = __builtin_amdgcn_fdot2(*reinterpret_cast<const h2_t*>(&q23), kOnes,
-                                                                     s01, false);
-                        }
-                    }
-                    *reinterpret_cast<float4*>(data + wo[m]) = make_float4(v[0], v[1], v[2], v[3]);
+                    *reinterpret_cast<float4*>(data + wo[m]) = group_sum_f16(al);
                 }
             } else {
                 float x[NI][kGroupCh][4];
@@ -1333,8 +1341,26 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
                 for (int q = 0; q < count; q++) {
                     const int g = g0 + q, c0 = kGroupCh * g, nchg = std::min(kGroupCh, nch - c0);
                     const int amin = dmin[(size_t)b * nch + c0];
-                    for (const auto& t : tup[g]) {
-                        const int rec[kRowInts] = {kGroupCh * q, amin, t[0], t[1], t[2], nchg, 0, 0};
+                    // One record per presum row: the byte offset of each channel's
+                    // first dword from the chunk's first channel row at time tile 0
+                    // (0xC0000000, past any window, for a missing channel), the
+                    // element phases (a byte each), the row's LDS byte offset and
+                    // the group's first channel in the chunk.  A tile adds only its
+                    // time offset and the lane's sample offset.
+                    for (size_t ri = 0; ri < tup[g].size(); ri++) {
+                        const auto& t = tup[g][ri];
+                        int rec[kRowInts] = {0, 0, 0, 0, 0, (rowstart + (int)ri) * stride * 4, kGroupCh * q, 0};
+                        for (int c = 0; c < kGroupCh; c++) {
+                            const int64_t e = (int64_t)amin + (c == 0 ? 0 : t[c - 1]);
+                            const int64_t bo = 4 * ((int64_t)(kGroupCh * q + c) * (p->ld / epd) + e / epd);
+                            // bases and the tile's time offset stay below 2^30 each, so
+                            // a missing channel's offset never wraps into the window
+                            VFRB_REQUIRE(bo < 0x40000000LL && (int64_t)p->ld * esz < 0x40000000LL,
+                                         VFRB_ERR_UNSUPPORTED, "channel row offset %lld too large for a chunk",
+                                         (long long)bo);
+                            rec[c] = c < nchg ? (int)bo : (int)0xC0000000u;
+                            rec[4] |= (int)(e % epd) << (8 * c);
+                        }
                         recs.insert(recs.end(), rec, rec + kRowInts);
                     }
                     for (int d = 0; d < DMB; d++) {
