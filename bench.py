@@ -680,10 +680,26 @@ def bench_c1(vfrb, args):
         ev1.sync()
         ms = ev0.elapsed_ms(ev1) / nsteps
         pk = peak.download()
+        # the same chain captured once (one segment's ring positions) and
+        # replayed: what launching the seven kernels one by one costs
+        g = vfrb.Graph(s)
+        g.begin()
+        step(3)
+        g.end()
+        g.launch()
+        ev0.record(s.h)
+        for _ in range(nsteps):
+            g.launch()
+        ev1.record(s.h)
+        ev1.sync()
+        gms = ev0.elapsed_ms(ev1) / nsteps
+        del g
         res[sname] = {"ms_per_segment": ms, "rtf": rows * tchan / (ms * 1e-3), "tile": vfrb.tile_names()[sp.tile],
-                      "best_dm": float(dms[int(np.argmax(pk))]), "best_snr": float(np.max(pk))}
+                      "best_dm": float(dms[int(np.argmax(pk))]), "best_snr": float(np.max(pk)),
+                      "graph_ms_per_segment": gms, "graph_note": "one segment's chain captured in a hipGraph and "
+                      "replayed (fixed ring positions): the launch cost of the eager chain is ms - graph_ms"}
         log(f"c1 {sname}: {ms:.3f} ms per 1 s segment, RTF {rows * tchan / (ms * 1e-3):.0f}, best DM "
-            f"{dms[int(np.argmax(pk))]:.1f} SNR {np.max(pk):.1f}")
+            f"{dms[int(np.argmax(pk))]:.1f} SNR {np.max(pk):.1f}; graph replay {gms:.3f} ms")
         del sp
     if not args.no_cpu_baseline:
         t0 = time.perf_counter()

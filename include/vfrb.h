@@ -134,6 +134,23 @@ vfrb_status vfrb_event_synchronize(void* event);
 vfrb_status vfrb_event_elapsed_ms(float* ms, void* start, void* stop);
 /* Work queued on `stream` after this call waits for `event` (cross-stream order). */
 vfrb_status vfrb_stream_wait_event(void* stream, void* event);
+/*
+ * hipGraph capture of any sequence of library calls (no counterpart in the
+ * reference, whose loop launches every kernel one by one):
+ * vfrb_graph_begin starts capturing the work later enqueued on `stream` (a
+ * stream of vfrb_stream_create, not the null stream); vfrb_graph_end stops
+ * it and instantiates the graph; vfrb_graph_launch replays it on a stream;
+ * vfrb_graph_destroy frees it.  A replay repeats the captured kernels with
+ * the captured arguments, so it suits calls whose pointers and sizes do not
+ * change.  The library's kernels clear their own counters with kernels, so
+ * a captured chain holds no memset node; vfrb_memset and synchronous copies
+ * do not belong in a capture.
+ */
+typedef struct vfrb_graph vfrb_graph;
+vfrb_status vfrb_graph_begin(void* stream);
+vfrb_status vfrb_graph_end(void* stream, vfrb_graph** graph);
+vfrb_status vfrb_graph_launch(vfrb_graph* graph, void* stream);
+vfrb_status vfrb_graph_destroy(vfrb_graph* graph);
 
 /* ------------------------------------------------------------------------ */
 /* Front end: fused unpack + R2C FFT + detection + pol sum + time average   */

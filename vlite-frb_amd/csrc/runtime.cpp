@@ -187,6 +187,46 @@ vfrb_status vfrb_stream_wait_event(void* stream, void* event) {
     return VFRB_OK;
 }
 
+struct vfrb_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+vfrb_status vfrb_graph_begin(void* stream) {
+    VFRB_REQUIRE(stream, VFRB_ERR_INVALID, "capture needs a created stream, not the null stream");
+    VFRB_HIP(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal));
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_graph_end(void* stream, vfrb_graph** graph) {
+    VFRB_REQUIRE(stream && graph, VFRB_ERR_INVALID, "NULL argument");
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    VFRB_HIP(hipStreamEndCapture(as_stream(stream), &g));
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return vfrb::hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    *graph = new vfrb_graph{g, x};
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_graph_launch(vfrb_graph* graph, void* stream) {
+    VFRB_REQUIRE(graph && graph->exec, VFRB_ERR_INVALID, "NULL graph");
+    VFRB_HIP(hipGraphLaunch(graph->exec, as_stream(stream)));
+    return VFRB_OK;
+}
+
+vfrb_status vfrb_graph_destroy(vfrb_graph* graph) {
+    if (!graph) return VFRB_OK;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return VFRB_OK;
+}
+
 vfrb_status vfrb_event_elapsed_ms(float* ms, void* start, void* stop) {
     VFRB_REQUIRE(ms, VFRB_ERR_INVALID, "ms is NULL");
     VFRB_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),

@@ -124,6 +124,10 @@ _SIGS = {
     "vfrb_event_synchronize": (i32, [vp]),
     "vfrb_event_elapsed_ms": (i32, [P(f32), vp, vp]),
     "vfrb_stream_wait_event": (i32, [vp, vp]),
+    "vfrb_graph_begin": (i32, [vp]),
+    "vfrb_graph_end": (i32, [vp, P(vp)]),
+    "vfrb_graph_launch": (i32, [vp, vp]),
+    "vfrb_graph_destroy": (i32, [vp]),
     "vfrb_fb_plan_create": (i32, [P(vp), i32, i32, i32, i32]),
     "vfrb_fb_plan_destroy": (i32, [vp]),
     "vfrb_fb_execute": (i32, [vp, vp, vp, i64, vp, i64, i64, vp]),
@@ -445,6 +449,30 @@ class Event:
         try:
             if self.h:
                 lib.vfrb_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """The library calls enqueued on `stream` between begin() and end(),
+    captured into a hipGraph and replayed by launch()."""
+
+    def __init__(self, stream: Stream):
+        self.stream, self.h = stream, vp()
+
+    def begin(self):
+        call("vfrb_graph_begin", self.stream.h)
+
+    def end(self):
+        call("vfrb_graph_end", self.stream.h, C.byref(self.h))
+
+    def launch(self, stream=None):
+        call("vfrb_graph_launch", self.h, _s(stream) if stream is not None else self.stream.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib.vfrb_graph_destroy(self.h)
         except Exception:
             pass
 
