@@ -1353,9 +1353,10 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
                         for (int c = 0; c < kGroupCh; c++) {
                             const int64_t e = (int64_t)amin + (c == 0 ? 0 : t[c - 1]);
                             const int64_t bo = 4 * ((int64_t)(kGroupCh * q + c) * (p->ld / epd) + e / epd);
-                            // bases and the tile's time offset stay below 2^30 each, so
-                            // a missing channel's offset never wraps into the window
-                            VFRB_REQUIRE(bo < 0x40000000LL && (int64_t)p->ld * esz < 0x40000000LL,
+                            // bases and the lane's offset (time tile + sample group, below
+                            // (ld + stride) elements) stay below 2^30 each, so a missing
+                            // channel's offset never wraps into the window
+                            VFRB_REQUIRE(bo < 0x40000000LL && ((int64_t)p->ld + stride) * esz < 0x40000000LL,
                                          VFRB_ERR_UNSUPPORTED, "channel row offset %lld too large for a chunk",
                                          (long long)bo);
                             rec[c] = c < nchg ? (int)bo : (int)0xC0000000u;
