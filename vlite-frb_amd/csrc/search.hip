@@ -934,13 +934,39 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
             }
         }
         __syncthreads();
-        for (int gi = 0; gi < ngr; gi++) {
-            const uint32_t* ro = roff + gi * DMB + wave * DW;
+        if constexpr (W <= 8 && DW % 2 == 0) {
+            // Tiles of 8 waves (4 per SIMD) keep two DMs' presum samples in
+            // flight before adding them: the compiler otherwise reads, waits
+            // and adds one pair of samples at a time, which these few waves
+            // cannot hide (G512_dm64_w8 6.0 -> 4.9 ms at 4096 ch x 2048 DM).
+            // 16-wave tiles hide it with their waves and keep the plain loop,
+            // which their 64 VGPRs hold without the buffer.
+            for (int gi = 0; gi < ngr; gi++) {
+                const uint32_t* ro = roff + gi * DMB + wave * DW;
 #pragma unroll
-            for (int d = 0; d < DW; d++) {
-                const float* src = reinterpret_cast<const float*>(data + ro[d]) + lane;
+                for (int d0 = 0; d0 < DW; d0 += 2) {
+                    float v[2][SL];
 #pragma unroll
-                for (int q = 0; q < SL; q++) acc[d][q] += src[64 * q];
+                    for (int e = 0; e < 2; e++) {
+                        const float* src = reinterpret_cast<const float*>(data + ro[d0 + e]) + lane;
+#pragma unroll
+                        for (int q = 0; q < SL; q++) v[e][q] = src[64 * q];
+                    }
+#pragma unroll
+                    for (int e = 0; e < 2; e++)
+#pragma unroll
+                        for (int q = 0; q < SL; q++) acc[d0 + e][q] += v[e][q];
+                }
+            }
+        } else {
+            for (int gi = 0; gi < ngr; gi++) {
+                const uint32_t* ro = roff + gi * DMB + wave * DW;
+#pragma unroll
+                for (int d = 0; d < DW; d++) {
+                    const float* src = reinterpret_cast<const float*>(data + ro[d]) + lane;
+#pragma unroll
+                    for (int q = 0; q < SL; q++) acc[d][q] += src[64 * q];
+                }
             }
         }
         __syncthreads();
