@@ -78,7 +78,7 @@ def test_impulse_response_full_size(gpu, full, storage):
     out = vfrb.DeviceArray((NDM, nt), np.float32)
     names = vfrb.tile_names()
     default = plan.tile
-    for tile in (default, 16, 12, 8, 3):
+    for tile in (default, 16, 20, 12, 8, 3):
         print(f"[case] impulse response {storage} tile {names[tile]}", flush=True)
         plan.set_tile(tile)
         out.zero()

@@ -192,7 +192,7 @@ def test_delay_table_device_matches_oracle(gpu, orc):
     assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
 
 
-@pytest.mark.parametrize("tile", range(20))
+@pytest.mark.parametrize("tile", range(24))
 def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()
@@ -232,7 +232,7 @@ def test_pair_tiles_match_channel_tiles(gpu, storage):
     plan.dedisperse(rows, out, nt)
     ref = out.download()
     names = vfrb.tile_names()
-    for i in (i for i, n in enumerate(names) if n[0] in "PG"):
+    for i in (i for i, n in enumerate(names) if n[0] in "PGW"):
         print(f"[case] pair tile {names[i]} {storage}", flush=True)
         plan.set_tile(i)
         out.zero()
@@ -267,7 +267,7 @@ def test_presum_tiles_ragged_channel_count(gpu, orc, storage, nch):
         oref = orc.dedisperse_f32(fb, plan.offsets(), nt)
         assert np.max(np.abs(ref - oref)) < 1e-4 * 22
     names = vfrb.tile_names()
-    for i in (i for i, n in enumerate(names) if n[0] in "PG"):
+    for i in (i for i, n in enumerate(names) if n[0] in "PGW"):
         print(f"[case] ragged {nch} ch, tile {names[i]} {storage}", flush=True)
         plan.set_tile(i)
         out.zero()
@@ -275,6 +275,36 @@ def test_presum_tiles_ragged_channel_count(gpu, orc, storage, nch):
         got = out.download()
         tol = (1e-3 if storage == "u8" else 1e-5) * np.max(np.abs(ref))
         assert np.max(np.abs(got - ref)) <= tol, (names[i], storage, nch, np.max(np.abs(got - ref)))
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_wide_group_tiles_equal_group_tiles(gpu, storage):
+    """The wide group tiles ("W") stage the same presum rows as the group
+    tiles ("G") and add them in the same order, reading four samples per lane
+    at a time: equal bit for bit, on a window whose DM starts fall on every
+    phase of a 4-sample quad, at a length that leaves a partial last tile."""
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(41)
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    names = vfrb.tile_names()
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[storage],
+                           names.index("G512_dm64_w16"))
+    nt = rows - plan.max_delay
+    phases = np.unique(plan.offsets()[:, ::4] % 4)
+    assert phases.size == 4, phases
+    d = vfrb.DeviceArray.from_host(fb)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.load(d, rows, bandpass=True)
+    plan.dedisperse(rows, out, nt)
+    ref = out.download()
+    for wide in (n for n in names if n[0] == "W"):
+        print(f"[case] wide tile {wide} {storage}", flush=True)
+        plan.set_tile(names.index(wide))
+        out.zero()
+        plan.dedisperse(rows, out, nt)
+        assert np.array_equal(out.download(), ref), wide
 
 
 def test_autotune_picks_a_fitting_tile_and_keeps_results(gpu, orc):

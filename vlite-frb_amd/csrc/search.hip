@@ -780,7 +780,116 @@ __device__ __forceinline__ float4 group_sum_f16(const uint32_t (&al)[kGroupCh][2
     return make_float4(v[0], v[1], v[2], v[3]);
 }
 
-template <typename TS, int SL, int DW, int W>
+// Samples per time tile: 64 per lane-sample for the per-channel, pair and
+// group tiles; 63 per lane-sample for the wide group tiles (MODE 3), whose
+// last lane only supplies its neighbour.
+__host__ __device__ constexpr int tile_samples(int sl, int mode) { return (mode == 3 ? 63 : 64) * sl; }
+
+// Wide group tiles.  Lane l holds presum quads Q+l (and Q+63+l) of a DM's
+// row and adds the DM's samples S+4l .. S+4l+3 (S = 4Q + R) of each to
+// a[4k..4k+3]: elements R..3 of its own quad and 0..R-1 of lane l+1's (DPP
+// wave_shl:1 on the add's first operand; lane 63 gets zeros, and its sums
+// are not used).  R is the same for the whole wave.  The branch over the four
+// phases sits inside one asm block: as four branches in C++ the compiler
+// merges them into one add sequence whose operands it selects, or copies the
+// sums in and out of each, and spills.  The s_nop gives a DPP operand written
+// by a VALU instruction its two wait states.
+template <int NB>
+__device__ __forceinline__ void wide_add(float* a, const float4 (&v)[NB], uint32_t r) {
+    static_assert(NB == 1 || NB == 2, "one or two quads per lane");
+    if constexpr (NB == 2)
+        asm volatile(
+        "s_bitcmp1_b32 %16, 1\n"
+        "s_cbranch_scc1 .Lwide_hi%=\n"
+        "s_bitcmp1_b32 %16, 0\n"
+        "s_cbranch_scc1 .Lwide_1%=\n"
+        "v_add_f32 %0, %8, %0\n"
+        "v_add_f32 %1, %9, %1\n"
+        "v_add_f32 %2, %10, %2\n"
+        "v_add_f32 %3, %11, %3\n"
+        "v_add_f32 %4, %12, %4\n"
+        "v_add_f32 %5, %13, %5\n"
+        "v_add_f32 %6, %14, %6\n"
+        "v_add_f32 %7, %15, %7\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_1%=:\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %9, %0\n"
+        "v_add_f32 %1, %10, %1\n"
+        "v_add_f32 %2, %11, %2\n"
+        "v_add_f32_dpp %3, %8, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32 %4, %13, %4\n"
+        "v_add_f32 %5, %14, %5\n"
+        "v_add_f32 %6, %15, %6\n"
+        "v_add_f32_dpp %7, %12, %7 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_hi%=:\n"
+        "s_bitcmp1_b32 %16, 0\n"
+        "s_cbranch_scc1 .Lwide_3%=\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %10, %0\n"
+        "v_add_f32 %1, %11, %1\n"
+        "v_add_f32_dpp %2, %8, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %3, %9, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32 %4, %14, %4\n"
+        "v_add_f32 %5, %15, %5\n"
+        "v_add_f32_dpp %6, %12, %6 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %7, %13, %7 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_3%=:\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %11, %0\n"
+        "v_add_f32_dpp %1, %8, %1 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %2, %9, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %3, %10, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32 %4, %15, %4\n"
+        "v_add_f32_dpp %5, %12, %5 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %6, %13, %6 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %7, %14, %7 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        ".Lwide_end%=:\n"
+        : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(a[4]), "+v"(a[5]), "+v"(a[6]), "+v"(a[7])
+        : "v"(v[0].x), "v"(v[0].y), "v"(v[0].z), "v"(v[0].w), "v"(v[1].x), "v"(v[1].y), "v"(v[1].z), "v"(v[1].w), "s"(r)
+        : "scc");
+    else
+        asm volatile(
+        "s_bitcmp1_b32 %8, 1\n"
+        "s_cbranch_scc1 .Lwide_hi%=\n"
+        "s_bitcmp1_b32 %8, 0\n"
+        "s_cbranch_scc1 .Lwide_1%=\n"
+        "v_add_f32 %0, %4, %0\n"
+        "v_add_f32 %1, %5, %1\n"
+        "v_add_f32 %2, %6, %2\n"
+        "v_add_f32 %3, %7, %3\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_1%=:\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %5, %0\n"
+        "v_add_f32 %1, %6, %1\n"
+        "v_add_f32 %2, %7, %2\n"
+        "v_add_f32_dpp %3, %4, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_hi%=:\n"
+        "s_bitcmp1_b32 %8, 0\n"
+        "s_cbranch_scc1 .Lwide_3%=\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %6, %0\n"
+        "v_add_f32 %1, %7, %1\n"
+        "v_add_f32_dpp %2, %4, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %3, %5, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "s_branch .Lwide_end%=\n"
+        ".Lwide_3%=:\n"
+        "s_nop 1\n"
+        "v_add_f32 %0, %7, %0\n"
+        "v_add_f32_dpp %1, %4, %1 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %2, %5, %2 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        "v_add_f32_dpp %3, %6, %3 wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:1\n"
+        ".Lwide_end%=:\n"
+        : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3])
+        : "v"(v[0].x), "v"(v[0].y), "v"(v[0].z), "v"(v[0].w), "s"(r)
+        : "scc");
+}
+
+template <typename TS, int SL, int DW, int W, bool WIDE = false>
 __device__ __forceinline__ void
 dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt, int ndm,
                   const uint32_t* __restrict__ roffs,  // per chunk: [groups][W*DW] LDS byte offsets
@@ -791,10 +900,12 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
                   unsigned char* lds_raw) {
     constexpr int ESZ = (int)sizeof(TS);
     constexpr int EPD = 4 / ESZ;
-    constexpr int T = 64 * SL;
+    constexpr int T = tile_samples(SL, WIDE ? 3 : 2);
     constexpr int DMB = W * DW;
     constexpr int NT = 64 * W;
     constexpr int NI = GroupStageItems<W>::value;
+    constexpr int NB = SL / 4;  // wide tiles: presum quads per lane and DM
+    static_assert(!WIDE || SL % 4 == 0, "wide tiles read whole quads");
     constexpr int NR = (kGroupMax * DMB / 4 + NT - 1) / NT;
     constexpr int NQ = (kRowMax * kRowInts + NT - 1) / NT;  // record ints per thread
     uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
@@ -934,7 +1045,31 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
             }
         }
         __syncthreads();
-        if constexpr (W <= 8 && DW % 2 == 0) {
+        if constexpr (WIDE) {
+            // Lane l reads quads Q+l and Q+63+l of a DM's row (Q = its start
+            // sample / 4) with 16-byte LDS reads, which run at nearly twice
+            // the rate of the two-sample reads below; wide_add picks the add
+            // sequence of the start's phase.  Lane 63's sums are not stored.
+            const uint32_t* ro = roff + wave * DW;
+            for (int gi = 0; gi < ngr; gi++) {
+                uint32_t o[DW];
+#pragma unroll
+                for (int d = 0; d < DW; d++) o[d] = __builtin_amdgcn_readfirstlane(ro[gi * DMB + d]);
+                // the next DM's quads are read before this one's are added
+                float4 v[2][NB];
+                auto quads = [&](int d) {
+                    const unsigned char* src = data + (o[d] & ~15u) + 16 * lane;
+#pragma unroll
+                    for (int k = 0; k < NB; k++) v[d & 1][k] = *reinterpret_cast<const float4*>(src + 16 * 63 * k);
+                };
+                quads(0);
+#pragma unroll
+                for (int d = 0; d < DW; d++) {
+                    if (d + 1 < DW) quads(d + 1);
+                    wide_add<NB>(acc[d], v[d & 1], (o[d] >> 2) & 3u);
+                }
+            }
+        } else if constexpr (W <= 8 && DW % 2 == 0) {
             // Tiles of 8 waves (4 per SIMD) keep two DMs' presum samples in
             // flight before adding them: the compiler otherwise reads, waits
             // and adds one pair of samples at a time, which these few waves
@@ -982,9 +1117,9 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
         float* o = out + (int64_t)dm * ld_out;
 #pragma unroll
         for (int q = 0; q < SL; q++) {
-            const int t = t0 + lane + 64 * q;
+            const int t = WIDE ? t0 + 252 * (q / 4) + 4 * lane + (q & 3) : t0 + lane + 64 * q;
             const float a = acc[d][q];
-            if (t < nt) {
+            if (t < nt && (!WIDE || lane < 63)) {
                 if constexpr (ESZ == 4) o[t] = a / fnch;
                 else if constexpr (ESZ == 2) o[t] = (a * mul) / fnch;
                 else o[t] = (a + add) / fnch;
@@ -1018,8 +1153,8 @@ dedisp_kernel(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int nt
     extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
     const int id = xcd_tile(blockIdx.x, gridDim.x);
     const int b = id % nblk, tx = id / nblk;
-    if constexpr (MODE == 2)
-        dedisp_group_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
+    if constexpr (MODE >= 2)
+        dedisp_group_tile<TS, SL, DW, W, MODE == 3>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
                                          chan_scale, f16_exp, offsum, out, ld_out, b, tx, lds_raw);
     else if constexpr (MODE == 1)
         dedisp_pair_tile<TS, SL, DW, W>(win, ld, nch, nrows, nt, ndm, roffs, dmin, chunks, blk_chunks,
@@ -1071,8 +1206,8 @@ dedisp_batch_kernel(const BatchJob* __restrict__ jobs, const BatchTile* __restri
         if (i >= total) break;
         const BatchTile tl = tiles[i];
         const BatchJob& j = jobs[tl.job];
-        if constexpr (MODE == 2)
-            dedisp_group_tile<TS, SL, DW, W>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
+        if constexpr (MODE >= 2)
+            dedisp_group_tile<TS, SL, DW, W, MODE == 3>(static_cast<const TS*>(j.win), j.ld, j.nch, j.nrows, j.nt, j.ndm,
                                              j.roffs, j.dmin, j.chunks, j.blk_chunks, j.chan_scale, j.f16_exp,
                                              j.offsum, j.out, j.ld_out, tl.blk, tl.tx, lds_raw);
         else if constexpr (MODE == 1)
@@ -1153,12 +1288,14 @@ __global__ void __launch_bounds__(256) stats_shift_kernel(const float* __restric
 // ---------------------------------------------------------------------------
 struct TileCfg {
     int sl, dw, w;
-    int mode;  // 0 per channel; 1 channel-pair presums ("P"); 2 4-channel group presums ("G")
+    int mode;  // 0 per channel; 1 channel-pair presums ("P"); 2 4-channel group presums ("G");
+               // 3 group presums read 16 bytes per lane ("W")
     const char* name;
 };
 // samples per lane (multiple of 4 so u8 lanes read whole dwords), DMs per
-// wave, waves per workgroup.  Tile = 64*SL samples x W*DW DMs.  "P" tiles sum
-// channel pairs first, "G" tiles groups of 4 channels.
+// wave, waves per workgroup.  Tile = 64*SL samples (W tiles: 63*SL) x W*DW
+// DMs.  "P" tiles sum channel pairs first, "G" and "W" tiles groups of 4
+// channels.
 constexpr TileCfg kTiles[] = {
     {4, 16, 4, 0, "T256_dm64_w4"},    {8, 16, 4, 0, "T512_dm64_w4"},
     {8, 8, 4, 0, "T512_dm32_w4"},     {8, 8, 8, 0, "T512_dm64_w8"},
@@ -1170,6 +1307,8 @@ constexpr TileCfg kTiles[] = {
     {4, 8, 16, 1, "P256_dm128_w16"},  {16, 4, 8, 1, "P1024_dm32_w8"},
     {8, 4, 16, 2, "G512_dm64_w16"},   {8, 8, 8, 2, "G512_dm64_w8"},
     {4, 8, 16, 2, "G256_dm128_w16"},  {4, 4, 16, 2, "G256_dm64_w16"},
+    {8, 4, 16, 3, "W504_dm64_w16"},   {8, 8, 8, 3, "W504_dm64_w8"},
+    {4, 8, 16, 3, "W252_dm128_w16"},  {4, 4, 16, 3, "W252_dm64_w16"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 // Default tile: the 4-channel group tile G256_dm64_w16.  Group and pair tiles
@@ -1290,7 +1429,7 @@ void free_tiles(TileSet& t) {
 // kernel reads: channel row + copy (delay mod samples-per-dword) + aligned
 // position of the delay inside that copy.
 vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, TileSet& out) {
-    const int T = 64 * tc.sl, DMB = tc.w * tc.dw;
+    const int T = tile_samples(tc.sl, tc.mode), DMB = tc.w * tc.dw;
     const int nch = p->nch, ndm = p->ndm;
     const int nblk = (ndm + DMB - 1) / DMB;
     const int esz = (int)storage_bytes(p->storage);
@@ -1321,7 +1460,7 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
             span[c] = hi - lo;
         }
         blk[b].x = (int)chunks.size();
-        if (tc.mode == 2) {
+        if (tc.mode >= 2) {
             // Group q = channels 4q..4q+3 (fewer in the last group).  Each
             // distinct tuple of relative delays of the group's channels over
             // the block's DMs is one presum row; a chunk's rows are listed as
@@ -1345,7 +1484,10 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
                     rowof[g].push_back(idx);
                 }
             }
-            auto gstride = [&](int sp) { return (T + sp + 3) / 4 * 4; };
+            // A row covers T + span samples; a wide tile's lane 63 reads up
+            // to 3 samples further (the quad after the last one used).
+            const int extra = tc.mode == 3 ? 4 : 0;
+            auto gstride = [&](int sp) { return (T + sp + extra + 3) / 4 * 4; };
             int g0 = 0;
             while (g0 < ngrp) {
                 int count = 0, rows = 0, smax = 0;
@@ -1504,7 +1646,7 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
         }
         blk[b].y = (int)chunks.size() - blk[b].x;
     }
-    if (tc.mode == 2) dmin.swap(recs);
+    if (tc.mode >= 2) dmin.swap(recs);
     if (dmin.empty()) dmin.push_back(0);
     free_tiles(out);
     hipError_t e = hipMalloc(&out.d_roffs, roffs.size() * sizeof(uint32_t));
@@ -1704,7 +1846,7 @@ vfrb_status vfrb_search_dedisperse(vfrb_search_plan* p, int32_t nrows, float* ou
     VFRB_REQUIRE(nrows <= p->loaded_rows, VFRB_ERR_INVALID,
                  "window holds %d rows, %d requested", p->loaded_rows, nrows);
     const TileCfg& tc = kTiles[p->ts.tile];
-    const int T = 64 * tc.sl;
+    const int T = tile_samples(tc.sl, tc.mode);
     const int64_t nwg = vfrb::ceil_div(nt, T) * p->ts.nblk;
     VFRB_REQUIRE(nwg < INT32_MAX, VFRB_ERR_UNSUPPORTED, "%lld dedispersion tiles", (long long)nwg);
     dim3 grid((unsigned)nwg), block(64 * tc.w);
@@ -1842,7 +1984,7 @@ vfrb_status batch_build(vfrb_batch* b, hipStream_t s) {
         b->staged_pending = false;
     }
     const TileCfg& tc = kTiles[b->tile];
-    const int T = 64 * tc.sl;
+    const int T = tile_samples(tc.sl, tc.mode);
     struct Blk {
         int64_t cost;
         int job, blk;
