@@ -749,15 +749,6 @@ dedisp_pair_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, int
 constexpr int kGroupCh = 4;
 constexpr int kGroupMax = kChunkMax / kGroupCh;
 
-template <int W>
-struct GroupStageItems {
-    static constexpr int groups = (int)(kDataBudget / 16);  // 16 B per item
-    static constexpr int need = (groups + 64 * W - 1) / (64 * W);
-    static constexpr int regs = kGroupCh * 4 + 4;
-    static constexpr int fit = (1024 / W / 4) / regs;
-    static constexpr int cap = fit < 1 ? 1 : (fit > 8 ? 8 : fit);
-    static constexpr int value = need < cap ? need : cap;
-};
 
 // f16 group presum of 4 samples: after the phase shift, channel c's dwords
 // hold elements (0,1) and (2,3).  v_perm pairs element e of two channels in
@@ -903,7 +894,6 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
     constexpr int T = tile_samples(SL, WIDE ? 3 : 2);
     constexpr int DMB = W * DW;
     constexpr int NT = 64 * W;
-    constexpr int NI = GroupStageItems<W>::value;
     constexpr int NB = SL / 4;  // wide tiles: presum quads per lane and DM
     static_assert(!WIDE || SL % 4 == 0, "wide tiles read whole quads");
     constexpr int NR = (kGroupMax * DMB / 4 + NT - 1) / NT;
@@ -934,7 +924,6 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
         const Chunk ch = uniform_load(chunks + cr.x + k);
         const int ngr = ch.count, nrw = ch.nd;
         const int ng = ch.stride / 4;  // 4-sample groups per row
-        const int items = nrw * ng;
         const int* rcur = rbuf + (k & 1) * kRowMax * kRowInts;
         const bool more = k + 1 < cr.y;
         Chunk nx = ch;
@@ -951,83 +940,95 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
         const int64_t left = ((int64_t)nch - ch.c0) * ld * ESZ;
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<TS*>(base), 0, (int)(left < 0x7fffffff ? left : 0x7fffffff), kBufferFlags);
-        const float rng = 1.0f / (float)ng;
         const uint32_t toff = 4u * (uint32_t)(t0 / EPD);  // t0 is a multiple of EPD
-        // Item j = (row r, 4-sample group g): the row's four channels' elements
-        // S + d_k + 4g .. +3 (buffer loads; zeros past the window or for a
-        // channel the last group lacks) summed into presum samples 4g..4g+3.
-        // The row record holds each channel's byte offset and phase.
-        auto stage = [&](int j0) {
-            uint32_t vo[NI][kGroupCh];
-            int ph[NI][kGroupCh], wo[NI];
-            float sc[NI][kGroupCh];
+        // Item j = (row r, 4-sample group g), j = r*ng + g: the row's four
+        // channels' elements S + d_k + 4g .. +3 (buffer loads; zeros past the
+        // window or for a channel the last group lacks) summed into presum
+        // samples 4g..4g+3.  The row record holds each channel's byte offset
+        // and phase.  A thread takes items tid, tid + NT, ...: (r, g) advance
+        // by (NT / ng, NT mod ng) with one carry, and a thread past the last
+        // row loads row 0's elements without writing them.
+        const int rq = NT / ng, rr = NT - rq * ng;
+        int r = (int)((float)tid * (1.0f / (float)ng));
+        int g = tid - r * ng;
+        {
+            const int fix = (g >> 31) + (g >= ng ? 1 : 0);
+            r += fix;
+            g -= fix * ng;
+        }
+        for (int j0 = 0; j0 < nrw * ng; j0 += NT) {
+            const bool ok = r < nrw;
+            const int* rec = rcur + (ok ? r : 0) * kRowInts;
+            const int4 rbase = *reinterpret_cast<const int4*>(rec);
+            const int4 rmeta = *reinterpret_cast<const int4*>(rec + 4);
+            const int bases[kGroupCh] = {rbase.x, rbase.y, rbase.z, rbase.w};
+            const uint32_t lo = toff + (uint32_t)(16 * g / EPD);
+            uint32_t vo[kGroupCh];
+            int ph[kGroupCh];
+            float sc[kGroupCh];
 #pragma unroll
-            for (int m = 0; m < NI; m++) {
-                const int j = j0 + tid + NT * m;
-                int r = (int)((float)j * rng);
-                int g = j - r * ng;
-                const int fix = (g >> 31) + (g >= ng ? 1 : 0);
-                r += fix;
-                g -= fix * ng;
-                const bool ok = j < items;
-                const int* rec = rcur + (ok ? r : 0) * kRowInts;
-                const int4 rbase = *reinterpret_cast<const int4*>(rec);
-                const int4 rmeta = *reinterpret_cast<const int4*>(rec + 4);
-                const int bases[kGroupCh] = {rbase.x, rbase.y, rbase.z, rbase.w};
-                const uint32_t lo = toff + (uint32_t)(16 * g / EPD);
-#pragma unroll
-                for (int c = 0; c < kGroupCh; c++) {
-                    vo[m][c] = ok ? (uint32_t)bases[c] + lo : 0x80000000u;
-                    ph[m][c] = (rmeta.x >> (8 * c)) & 3;
-                    sc[m][c] = 1.0f;
-                    if constexpr (ESZ == 1) sc[m][c] = bases[c] >= 0 ? chan_scale[ch.c0 + rmeta.z + c] : 0.0f;
-                }
-                wo[m] = ok ? rmeta.y + 16 * g : -1;
+            for (int c = 0; c < kGroupCh; c++) {
+                vo[c] = (uint32_t)bases[c] + lo;
+                ph[c] = (rmeta.x >> (8 * c)) & 3;
+                sc[c] = 1.0f;
+                if constexpr (ESZ == 1) sc[c] = bases[c] >= 0 ? chan_scale[ch.c0 + rmeta.z + c] : 0.0f;
             }
+            unsigned char* dst = data + rmeta.y + 16 * g;
             if constexpr (ESZ == 2) {
                 // f16: three raw dwords per channel, shifted by the phase and
                 // summed by group_sum_f16
-                uint32_t raw[NI][kGroupCh][3];
+                uint32_t raw[kGroupCh][3];
 #pragma unroll
-                for (int m = 0; m < NI; m++)
+                for (int c = 0; c < kGroupCh; c++)
 #pragma unroll
-                    for (int c = 0; c < kGroupCh; c++)
+                    for (int i = 0; i < 3; i++)
+                        raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
+                // all twelve loads in flight before the first use: the
+                // scheduler otherwise waits for each channel in turn
+                __builtin_amdgcn_sched_barrier(0);
+                uint32_t al[kGroupCh][2];
 #pragma unroll
-                        for (int i = 0; i < 3; i++)
-                            raw[m][c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[m][c] + 4 * i, 0, 0);
+                for (int c = 0; c < kGroupCh; c++)
 #pragma unroll
-                for (int m = 0; m < NI; m++) {
-                    if (wo[m] < 0) continue;
-                    uint32_t al[kGroupCh][2];
-#pragma unroll
-                    for (int c = 0; c < kGroupCh; c++)
-#pragma unroll
-                        for (int i = 0; i < 2; i++)
-                            al[c][i] = __builtin_amdgcn_alignbyte(raw[m][c][i + 1], raw[m][c][i], 2 * ph[m][c]);
-                    *reinterpret_cast<float4*>(data + wo[m]) = group_sum_f16(al);
-                }
+                    for (int i = 0; i < 2; i++) al[c][i] = __builtin_amdgcn_alignbyte(raw[c][i + 1], raw[c][i], 2 * ph[c]);
+                if (ok) *reinterpret_cast<float4*>(dst) = group_sum_f16(al);
             } else {
-                float x[NI][kGroupCh][4];
+                // f32: four dwords per channel; u8: two, shifted by the phase
+                // and scaled (the elements of load_elems<TS, 4>), all loads
+                // in flight before the first use
+                constexpr int NW = ESZ == 4 ? 4 : 2;
+                uint32_t raw[kGroupCh][NW];
 #pragma unroll
-                for (int m = 0; m < NI; m++)
+                for (int c = 0; c < kGroupCh; c++)
 #pragma unroll
-                    for (int c = 0; c < kGroupCh; c++)
-                        load_elems<TS, 4>(rsrc, vo[m][c], ph[m][c], sc[m][c], x[m][c]);
+                    for (int i = 0; i < NW; i++)
+                        raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+                float x[kGroupCh][4];
 #pragma unroll
-                for (int m = 0; m < NI; m++) {
-                    if (wo[m] < 0) continue;
-                    float4 w;
-                    w.x = (x[m][0][0] + x[m][1][0]) + (x[m][2][0] + x[m][3][0]);
-                    w.y = (x[m][0][1] + x[m][1][1]) + (x[m][2][1] + x[m][3][1]);
-                    w.z = (x[m][0][2] + x[m][1][2]) + (x[m][2][2] + x[m][3][2]);
-                    w.w = (x[m][0][3] + x[m][1][3]) + (x[m][2][3] + x[m][3][3]);
-                    *reinterpret_cast<float4*>(data + wo[m]) = w;
+                for (int c = 0; c < kGroupCh; c++) {
+                    if constexpr (ESZ == 4) {
+#pragma unroll
+                        for (int e = 0; e < 4; e++) x[c][e] = __uint_as_float(raw[c][e]);
+                    } else {
+                        const uint32_t wd = __builtin_amdgcn_alignbyte(raw[c][1], raw[c][0], ph[c]);
+#pragma unroll
+                        for (int e = 0; e < 4; e++) x[c][e] = sc[c] * (float)((wd >> (8 * e)) & 0xffu);
+                    }
                 }
+                float4 w;
+                w.x = (x[0][0] + x[1][0]) + (x[2][0] + x[3][0]);
+                w.y = (x[0][1] + x[1][1]) + (x[2][1] + x[3][1]);
+                w.z = (x[0][2] + x[1][2]) + (x[2][2] + x[3][2]);
+                w.w = (x[0][3] + x[1][3]) + (x[2][3] + x[3][3]);
+                if (ok) *reinterpret_cast<float4*>(dst) = w;
             }
-        };
-        stage(0);
-        if constexpr (GroupStageItems<W>::need > GroupStageItems<W>::value) {
-            for (int j0 = NT * NI; j0 < items; j0 += NT * NI) stage(j0);
+            r += rq;
+            g += rr;
+            if (g >= ng) {
+                g -= ng;
+                r++;
+            }
         }
         // the chunk's offset table, and the next chunk's row records
 #pragma unroll
