@@ -534,6 +534,23 @@ vfrb_status vfrb_server_drain(vfrb_server* server);
 vfrb_status vfrb_server_get_stats(const vfrb_server* server, vfrb_server_stats* stats);
 /* Server clock: seconds since vfrb_server_create (steady clock). */
 double      vfrb_server_now(const vfrb_server* server);
+/*
+ * Stream state (no reference counterpart: the reference keeps its overlap
+ * tail powLEFTOVER in memory only, PIPELINE:463-464,653,696-701).  A stream's
+ * state is the raw filterbank rows its next window starts with (the last
+ * max_delay rows, f32), the payload bytes short of a row, and its row count.
+ * save waits until the stream's submitted segments have finished, then
+ * copies the state to host memory: *nbytes receives its size, buf NULL only
+ * asks for it, VFRB_ERR_CAPACITY when cap is smaller.  restore puts a saved
+ * state into a stream (of this or another server) opened with the same
+ * filterbank shape, band, sample time and DM list (VFRB_ERR_INVALID
+ * otherwise); its next segments then give the results the saved stream's
+ * would.  Do not submit to the stream during either call.
+ */
+vfrb_status vfrb_server_save_stream(vfrb_server* server, int32_t stream_id, void* buf, int64_t cap,
+                                    int64_t* nbytes);
+vfrb_status vfrb_server_restore_stream(vfrb_server* server, int32_t stream_id, const void* buf,
+                                       int64_t nbytes);
 
 /* ------------------------------------------------------------------------ */
 /* Per-DM statistics and candidates                                         */

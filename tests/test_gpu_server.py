@@ -15,12 +15,12 @@ def _shape(vfrb):
     return vfrb.params_derive(N, NAVE, 1, 1.0, fullband=True)
 
 
-def _stream_bytes(vfrb, p, seed, pulse_t0):
+def _stream_bytes(vfrb, p, seed, pulse_t0, snr=30, fwhm=0.002):
     nbytes = [int(r * N * NAVE) for r in SEG_ROWS]
     total = sum(nbytes)
     nblk = -(-total // N)
-    amp = vfrb.amp_for_snr(30, NAVE, p.nchanout)
-    spec = vfrb.synth_spec(N, seed=seed, pulses=[(2.0, pulse_t0, 0.002, amp)])
+    amp = vfrb.amp_for_snr(snr, NAVE, p.nchanout)
+    spec = vfrb.synth_spec(N, seed=seed, pulses=[(2.0, pulse_t0, fwhm, amp)])
     p0 = vfrb.synth_payload(spec, 0, 0, nblk)[:total]
     p1 = vfrb.synth_payload(spec, 1, 0, nblk)[:total]
     cuts = np.concatenate([[0], np.cumsum(nbytes)])
@@ -67,7 +67,8 @@ def test_server_windows_match_oracle_replay(gpu, orc):
     vfrb = gpu
     p = _shape(vfrb)
     print("[case] server f32 plain vs oracle replay", flush=True)
-    p0, p1, cuts = _stream_bytes(vfrb, p, seed=3, pulse_t0=0.15)
+    # a narrow, strong pulse, so that the last window's best DM and time are checked
+    p0, p1, cuts = _stream_bytes(vfrb, p, seed=3, pulse_t0=0.19, snr=120, fwhm=0.0003)
     srv = _server(vfrb, p, vfrb.STORE_F32, batched=False)
     sid = srv.open_stream(DMS)
     maxd = srv.max_delay(sid)
@@ -79,7 +80,7 @@ def test_server_windows_match_oracle_replay(gpu, orc):
     off, maxd_o = orc.delay_table(DMS, p.nchanout, p.fch1, p.chanbw, p.tchan)
     assert maxd == maxd_o
     prev = 0
-    searched = 0
+    searched = pulsed = 0
     for k in range(len(SEG_ROWS)):
         r = res[k]
         tot = int(cuts[k + 1] // rb)
@@ -101,7 +102,8 @@ def test_server_windows_match_oracle_replay(gpu, orc):
         b = int(np.argmax(peak))
         if peak[b] > 10:  # the pulse is inside this window
             assert r["best_dm"] == b and r["best_t"] == pt[b] + tot - W
-    assert searched >= 4
+            pulsed += 1
+    assert searched >= 4 and pulsed == 1
 
 
 def test_server_result_is_the_same_alone_and_in_the_mix(gpu):
@@ -333,3 +335,70 @@ def test_server_device_input_in_place_equals_copy(gpu):
             if host[k]["nt"]:
                 for key in ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr"):
                     assert np.array_equal(host[k][key], dev[k][key]), (k, key)
+
+
+def _collect(srv, n):
+    srv.drain()
+    res = {}
+    while len(res) < n:
+        got = srv.poll(wait_ms=0)
+        assert got, f"{len(res)} of {n} results"
+        for r in got:
+            res[r["tag"]] = r
+    return res
+
+
+def test_server_stream_state_save_and_restore(gpu):
+    """A stream's state saved after two segments (a partial row carried) and
+    restored into a stream of a new server (batched, another segment limit,
+    so another ring length) continues as the saved stream does: the next
+    four segments give the same windows, per-DM results and candidates bit
+    for bit.  A state restores only into a stream with the same DM list and
+    band; a truncated state and a short buffer are refused."""
+    vfrb = gpu
+    p = _shape(vfrb)
+    # a short window's own spread caps a pulse's SNR near sqrt(nt / width):
+    # a narrow, strong pulse clears the 7-sigma threshold in the last window
+    p0, p1, cuts = _stream_bytes(vfrb, p, seed=5, pulse_t0=0.19, snr=120, fwhm=0.0003)
+    nseg, k_save = len(SEG_ROWS), 2
+    rb = N * NAVE
+    assert cuts[k_save] % rb != 0, "a partial row is carried at the save point"
+    print("[case] server stream state: uninterrupted stream", flush=True)
+    a = _server(vfrb, p, vfrb.STORE_F32, batched=False)
+    sa = a.open_stream(DMS)
+    for k in range(k_save):
+        a.submit(sa, p0[cuts[k]:cuts[k + 1]], p1[cuts[k]:cuts[k + 1]], k)
+    state = a.save_stream(sa)
+    maxd = a.max_delay(sa)
+    assert len(state) > maxd * p.nchanout * 4
+    for k in range(k_save, nseg):
+        a.submit(sa, p0[cuts[k]:cuts[k + 1]], p1[cuts[k]:cuts[k + 1]], k)
+    ref = _collect(a, nseg)
+    print("[case] server stream state: restored into a new server", flush=True)
+    b = _server(vfrb, p, vfrb.STORE_F32, batched=True, max_segment_bytes=12 << 20)
+    sb = b.open_stream(DMS)
+    other = b.open_stream(DMS[:-1])
+    with pytest.raises(vfrb.VfrbError):
+        b.restore_stream(other, state)
+    with pytest.raises(vfrb.VfrbError):
+        b.restore_stream(sb, state[:-4])
+    with pytest.raises(vfrb.VfrbError):
+        vfrb.call("vfrb_server_save_stream", b.h, sb, vfrb.C.create_string_buffer(16), 16,
+                  vfrb.C.byref(vfrb.i64(0)))
+    b.restore_stream(sb, state)
+    for k in range(k_save, nseg):
+        b.submit(sb, p0[cuts[k]:cuts[k + 1]], p1[cuts[k]:cuts[k + 1]], k)
+    got = _collect(b, nseg - k_save)
+    n_cand = 0
+    for k in range(k_save, nseg):
+        x, y = got[k], ref[k]
+        for f in ("status", "rows", "sample0", "nt", "ncand", "best_dm", "best_t", "best_snr"):
+            assert x[f] == y[f], (k, f, x[f], y[f])
+        for f in ("peak", "peak_t", "ave", "sigma", "cand_dm", "cand_t", "cand_snr"):
+            assert np.array_equal(x[f], y[f]), (k, f)
+        n_cand += x["ncand"]
+    assert n_cand > 0, ("the pulse after the save point gives candidates", [ref[k]["best_snr"] for k in ref])
+    # the state saved now from the restored stream equals the uninterrupted one's
+    assert b.save_stream(sb) == a.save_stream(sa)
+    a.close()
+    b.close()

@@ -114,6 +114,7 @@ struct SrvStream {
     unsigned char* d_res = nullptr;
     unsigned char* h_res = nullptr;  // pinned
     bool busy = false;
+    uint64_t key = 0;  // filterbank shape + delay parameters + DM list (saved states carry it)
 };
 
 namespace {
@@ -618,6 +619,23 @@ vfrb_status vfrb_server_destroy(vfrb_server* S) {
     return VFRB_OK;
 }
 
+// FNV-1a over the values that give a stream's saved state its meaning: the
+// filterbank shape and the delays (band, sample time, DM list).
+static uint64_t stream_key(const vfrb_server_desc& d, const float* dms, int32_t ndm) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    };
+    const int32_t shape[4] = {d.twonchan, d.nave, d.nchans, d.nskip};
+    const double band[3] = {d.fch1, d.chanbw, d.tsamp};
+    mix(shape, sizeof shape);
+    mix(band, sizeof band);
+    mix(&ndm, sizeof ndm);
+    mix(dms, sizeof(float) * (size_t)ndm);
+    return h;
+}
+
 vfrb_status vfrb_server_open_stream(vfrb_server* S, const float* dms, int32_t ndm, int32_t* stream_id) {
     VFRB_REQUIRE(S && dms && ndm > 0 && stream_id, VFRB_ERR_INVALID, "bad arguments");
     {
@@ -635,6 +653,7 @@ vfrb_status vfrb_server_open_stream(vfrb_server* S, const float* dms, int32_t nd
     SrvStream* s = ss.get();
     s->ndm = ndm;
     s->max_delay = max_delay;
+    s->key = stream_key(d, dms, ndm);
     s->ring_rows = max_delay + S->max_seg_rows + 1;
     s->ld_out = (S->max_seg_rows + 1 + 63) / 64 * 64;
     s->lay.set(ndm, d.max_cand);
@@ -766,5 +785,154 @@ vfrb_status vfrb_server_get_stats(const vfrb_server* S, vfrb_server_stats* out) 
 }
 
 double vfrb_server_now(const vfrb_server* S) { return S ? S->now() : 0.0; }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Stream state: the overlap rows (the raw filterbank rows the next window
+// starts with, f32), the payload bytes short of a row, and the row count.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StreamStateHeader {
+    char magic[8];
+    int32_t version, nchans, ndm, max_delay;
+    uint64_t key;
+    int64_t total_rows, keep_rows, carry_bytes, row_bytes;
+};
+constexpr char kStateMagic[8] = {'V', 'F', 'R', 'B', 'S', 'T', 'R', 'M'};
+constexpr int32_t kStateVersion = 1;
+
+int64_t keep_rows(const SrvStream* st) { return std::min<int64_t>(st->total_rows, st->max_delay); }
+
+int64_t state_bytes(const vfrb_server* S, int64_t keep, int64_t carry) {
+    return (int64_t)sizeof(StreamStateHeader) + (2 * carry + 3) / 4 * 4 + keep * S->d.nchans * 4;
+}
+
+// Waits (lock held) until stream id has no segment queued or in flight, then
+// marks it busy so that the scheduler leaves it alone while its state moves.
+vfrb_status claim_idle(vfrb_server* S, int32_t id, SrvStream** out) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    VFRB_REQUIRE(id >= 0 && id < (int)S->streams.size(), VFRB_ERR_INVALID, "stream %d not open", id);
+    SrvStream* st = S->streams[id].get();
+    S->cv_done.wait(lk, [&] {
+        if (st->busy) return false;
+        for (const Request& r : S->pending)
+            if (r.stream == id) return false;
+        return true;
+    });
+    st->busy = true;
+    *out = st;
+    return VFRB_OK;
+}
+
+void release(vfrb_server* S, SrvStream* st) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        st->busy = false;
+    }
+    S->cv_sched.notify_one();
+    S->cv_done.notify_all();
+}
+
+// Copies rows [r0, r0 + n) of a stream's ring (row r at r mod ring_rows)
+// to or from host rows, in at most two contiguous pieces.
+hipError_t ring_copy(SrvStream* st, int nch, int64_t r0, int64_t n, float* host, bool to_host) {
+    int64_t done = 0;
+    while (done < n) {
+        const int64_t pos = (r0 + done) % st->ring_rows;
+        const int64_t len = std::min(n - done, st->ring_rows - pos);
+        float* dev = st->ring + pos * nch;
+        float* h = host + done * nch;
+        const size_t bytes = (size_t)len * nch * sizeof(float);
+        const hipError_t e = to_host ? hipMemcpy(h, dev, bytes, hipMemcpyDeviceToHost)
+                                     : hipMemcpy(dev, h, bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return e;
+        done += len;
+    }
+    return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+vfrb_status vfrb_server_save_stream(vfrb_server* S, int32_t stream_id, void* buf, int64_t cap, int64_t* nbytes) {
+    VFRB_REQUIRE(S && nbytes, VFRB_ERR_INVALID, "NULL argument");
+    SrvStream* st = nullptr;
+    vfrb_status rc = claim_idle(S, stream_id, &st);
+    if (rc != VFRB_OK) return rc;
+    const int64_t keep = keep_rows(st), carry = st->carry_bytes;
+    const int64_t need = state_bytes(S, keep, carry);
+    *nbytes = need;
+    if (buf && cap < need) {
+        release(S, st);
+        VFRB_REQUIRE(false, VFRB_ERR_CAPACITY, "stream state needs %lld bytes, buffer holds %lld",
+                     (long long)need, (long long)cap);
+    }
+    if (buf) {
+        StreamStateHeader h{};
+        std::memcpy(h.magic, kStateMagic, sizeof h.magic);
+        h.version = kStateVersion;
+        h.nchans = S->d.nchans;
+        h.ndm = st->ndm;
+        h.max_delay = st->max_delay;
+        h.key = st->key;
+        h.total_rows = st->total_rows;
+        h.keep_rows = keep;
+        h.carry_bytes = carry;
+        h.row_bytes = S->row_bytes;
+        unsigned char* b = static_cast<unsigned char*>(buf);
+        std::memcpy(b, &h, sizeof h);
+        unsigned char* cb = b + sizeof h;
+        hipError_t e = hipSuccess;
+        for (int p = 0; p < 2 && e == hipSuccess && carry > 0; p++)
+            e = hipMemcpy(cb + p * carry, st->carry + p * S->row_bytes, (size_t)carry, hipMemcpyDeviceToHost);
+        float* rows = reinterpret_cast<float*>(cb + (2 * carry + 3) / 4 * 4);
+        if (e == hipSuccess) e = ring_copy(st, S->d.nchans, st->total_rows - keep, keep, rows, true);
+        if (e != hipSuccess) rc = vfrb::hip_fail(e, "vfrb_server_save_stream", __FILE__, __LINE__);
+    }
+    release(S, st);
+    return rc;
+}
+
+vfrb_status vfrb_server_restore_stream(vfrb_server* S, int32_t stream_id, const void* buf, int64_t nbytes) {
+    VFRB_REQUIRE(S && buf, VFRB_ERR_INVALID, "NULL argument");
+    VFRB_REQUIRE(nbytes >= (int64_t)sizeof(StreamStateHeader), VFRB_ERR_INVALID,
+                 "%lld bytes hold no stream state", (long long)nbytes);
+    StreamStateHeader h;
+    std::memcpy(&h, buf, sizeof h);
+    VFRB_REQUIRE(std::memcmp(h.magic, kStateMagic, sizeof h.magic) == 0 && h.version == kStateVersion,
+                 VFRB_ERR_INVALID, "not a stream state of this version");
+    VFRB_REQUIRE(h.total_rows >= 0 && h.keep_rows == std::min<int64_t>(h.total_rows, h.max_delay) &&
+                     h.carry_bytes >= 0 && h.carry_bytes < h.row_bytes,
+                     VFRB_ERR_INVALID, "inconsistent stream state");
+    VFRB_REQUIRE(nbytes == state_bytes(S, h.keep_rows, h.carry_bytes) && h.nchans == S->d.nchans &&
+                     h.row_bytes == S->row_bytes,
+                 VFRB_ERR_INVALID, "stream state of another filterbank shape or truncated");
+    SrvStream* st = nullptr;
+    vfrb_status rc = claim_idle(S, stream_id, &st);
+    if (rc != VFRB_OK) return rc;
+    if (h.key != st->key || h.ndm != st->ndm || h.max_delay != st->max_delay) {
+        release(S, st);
+        VFRB_REQUIRE(false, VFRB_ERR_INVALID, "stream state of another band or DM list");
+    }
+    const unsigned char* cb = static_cast<const unsigned char*>(buf) + sizeof h;
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < 2 && e == hipSuccess && h.carry_bytes > 0; p++)
+        e = hipMemcpy(st->carry + p * S->row_bytes, cb + p * h.carry_bytes, (size_t)h.carry_bytes,
+                      hipMemcpyHostToDevice);
+    const float* rows = reinterpret_cast<const float*>(cb + (2 * h.carry_bytes + 3) / 4 * 4);
+    if (e == hipSuccess)
+        e = ring_copy(st, S->d.nchans, h.total_rows - h.keep_rows, h.keep_rows, const_cast<float*>(rows), false);
+    if (e == hipSuccess) {
+        st->total_rows = h.total_rows;
+        st->carry_bytes = h.carry_bytes;
+    } else {
+        rc = vfrb::hip_fail(e, "vfrb_server_restore_stream", __FILE__, __LINE__);
+    }
+    release(S, st);
+    return rc;
+}
 
 }  // extern "C"

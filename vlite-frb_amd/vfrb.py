@@ -173,6 +173,8 @@ _SIGS = {
     "vfrb_server_drain": (i32, [vp]),
     "vfrb_server_get_stats": (i32, [vp, P(ServerStats)]),
     "vfrb_server_now": (f64, [vp]),
+    "vfrb_server_save_stream": (i32, [vp, i32, vp, i64, P(i64)]),
+    "vfrb_server_restore_stream": (i32, [vp, i32, vp, i64]),
     "vfrb_dm_stats": (i32, [vp, i64, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
     "vfrb_candidates": (i32, [vp, i64, i32, i32, vp, vp, f32, vp, vp, vp, i32, vp, vp]),
     "vfrb_vdif_decode": (i32, [P(C.c_uint8), P(VdifHeader)]),
@@ -752,6 +754,18 @@ class Server:
 
     def drain(self):
         call("vfrb_server_drain", self.h)
+
+    def save_stream(self, sid) -> bytes:
+        """The stream's state (overlap rows, partial row, row count) once its
+        submitted segments have finished."""
+        n = i64(0)
+        call("vfrb_server_save_stream", self.h, sid, None, 0, C.byref(n))
+        buf = C.create_string_buffer(n.value)
+        call("vfrb_server_save_stream", self.h, sid, buf, n.value, C.byref(n))
+        return buf.raw[: n.value]
+
+    def restore_stream(self, sid, state: bytes):
+        call("vfrb_server_restore_stream", self.h, sid, state, len(state))
 
     def stats(self):
         st = ServerStats()
