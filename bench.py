@@ -1089,7 +1089,7 @@ def bench_serving_full(vfrb, args, ctx):
     nstreams, per_stream = 8, 3
     seg_s = seg_rows * tchan
     # the tile the autotuner keeps on 20 s windows at this shape, for both servers
-    tile = vfrb.tile_names().index("G512_dm64_w16")
+    tile = vfrb.tile_names().index("W504_dm64_w16")
 
     def make(batched, storage):
         srv = vfrb.Server(n, nave, nch, p.nskip, p.fch1, p.chanbw, tchan, storage=storage,
@@ -1126,7 +1126,7 @@ def bench_serving_full(vfrb, args, ctx):
 
     modes = (("fp16_batched", True, vfrb.STORE_F16), ("fp32_plain", False, vfrb.STORE_F32))
     res = {"workload": f"{nstreams} streams x {per_stream} segments of {seg_s:.2f} s, 4096 ch x 2048 DM "
-                       "(0-1000), device-resident payload read in place, tile G512_dm64_w16",
+                       "(0-1000), device-resident payload read in place, tile W504_dm64_w16",
            "closed_loop": {}}
     for name, batched, storage in modes:
         srv, ids = make(batched, storage)
