@@ -44,12 +44,16 @@ def _batch_results(jobs):
     return [out.download()[:, : rows - plan.max_delay] for plan, rows, out, _ in jobs]
 
 
+@pytest.mark.parametrize("tile", [None, "W504_dm64_w16", "W252_dm64_w16"])
 @pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
-def test_batch_equals_single_plans(gpu, orc, storage):
+def test_batch_equals_single_plans(gpu, orc, storage, tile):
+    """The default batch (group tile) and batches fixed to the wide group
+    tiles: bit-identical to each plan run alone on the batch's tile."""
     vfrb = gpu
-    print(f"[case] batch vs single plans storage={storage}", flush=True)
+    print(f"[case] batch vs single plans storage={storage} tile={tile}", flush=True)
     jobs, fbs = _make_jobs(vfrb, storage, seed=11)
-    batch = vfrb.Batch(vfrb.STORAGE_NAMES[storage])
+    t = -1 if tile is None else vfrb.tile_names().index(tile)
+    batch = vfrb.Batch(vfrb.STORAGE_NAMES[storage], tile=t)
     ntiles = batch.set(jobs)
     assert ntiles > 0 and batch.grid > 0
     for _, _, out, _ in jobs:
