@@ -192,7 +192,7 @@ def test_delay_table_device_matches_oracle(gpu, orc):
     assert np.array_equal(plan.offsets(), off) and plan.max_delay == maxd == 308
 
 
-@pytest.mark.parametrize("tile", range(24))
+@pytest.mark.parametrize("tile", range(26))
 def test_dedisperse_f32_all_tiles(gpu, orc, tile):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()

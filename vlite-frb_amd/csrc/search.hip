@@ -1310,6 +1310,7 @@ constexpr TileCfg kTiles[] = {
     {4, 8, 16, 2, "G256_dm128_w16"},  {4, 4, 16, 2, "G256_dm64_w16"},
     {8, 4, 16, 3, "W504_dm64_w16"},   {8, 8, 8, 3, "W504_dm64_w8"},
     {4, 8, 16, 3, "W252_dm128_w16"},  {4, 4, 16, 3, "W252_dm64_w16"},
+    {4, 4, 8, 2, "G256_dm32_w8"},     {4, 2, 8, 2, "G256_dm16_w8"},
 };
 constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
 // Default tile: the 4-channel group tile G256_dm64_w16.  Group and pair tiles
