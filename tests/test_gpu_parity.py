@@ -307,6 +307,47 @@ def test_wide_group_tiles_equal_group_tiles(gpu, storage):
         assert np.array_equal(out.download(), ref), wide
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_presum_tiles_irregular_dm_list(gpu, orc, storage):
+    """200 DMs (a partial last DM block for every tile) at irregular spacing
+    (a sorted random list, two DMs repeated), a window whose output length is
+    not a multiple of any tile: every pair, group and wide tile that fits the
+    list equals the per-channel tile to f32 rounding, the wide tiles equal the
+    group tiles bit for bit, and in f32 the oracle's channel mean."""
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, _ = _c1_shape()
+    rng = np.random.default_rng(77)
+    dms = np.sort(np.concatenate([rng.uniform(0.0, 25.0, 198), [3.0, 3.0]])).astype(np.float32)
+    rows = 901
+    fb = rng.normal(0, 22, (rows, nch)).astype(np.float32)
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[storage], 3)
+    nt = rows - plan.max_delay
+    assert all(nt % t for t in (252, 256, 504, 512))
+    d = vfrb.DeviceArray.from_host(fb)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    plan.load(d, rows, bandpass=False)
+    plan.dedisperse(rows, out, nt)
+    ref = out.download()
+    if storage == "f32":
+        oref = orc.dedisperse_f32(fb, plan.offsets(), nt)
+        assert np.max(np.abs(ref - oref)) < 1e-4 * 22
+    names = vfrb.tile_names()
+    got = {}
+    for i in (i for i, n in enumerate(names) if n[0] in "PGW"):
+        try:
+            plan.set_tile(i)
+        except vfrb.VfrbError:
+            continue  # the list is too coarse for this tile
+        print(f"[case] irregular DM list, tile {names[i]} {storage}", flush=True)
+        out.zero()
+        plan.dedisperse(rows, out, nt)
+        got[names[i]] = out.download()
+        assert np.max(np.abs(got[names[i]] - ref)) <= 1e-5 * np.max(np.abs(ref)), names[i]
+    for w, g in (("W504_dm64_w16", "G512_dm64_w16"), ("W252_dm64_w16", "G256_dm64_w16")):
+        assert w in got and g in got
+        assert np.array_equal(got[w], got[g]), (w, g)
+
+
 def test_autotune_picks_a_fitting_tile_and_keeps_results(gpu, orc):
     vfrb = gpu
     nch, tchan, fch1, chanbw, dms = _c1_shape()
