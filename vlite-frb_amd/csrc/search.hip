@@ -896,7 +896,10 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
     constexpr int NT = 64 * W;
     constexpr int NB = SL / 4;  // wide tiles: presum quads per lane and DM
     static_assert(!WIDE || SL % 4 == 0, "wide tiles read whole quads");
-    constexpr int NR = (kGroupMax * DMB / 4 + NT - 1) / NT;
+    constexpr int OPW = WIDE ? 2 : 1;  // DM offsets per table word (wide tiles: two 16-bit halves)
+    static_assert(!WIDE || (DW % 2 == 0 && DMB % 8 == 0), "wide tiles pair the DMs of a wave");
+    static_assert(kDataBudget < 65536, "16-bit LDS offsets");
+    constexpr int NR = (kGroupMax * DMB / (4 * OPW) + NT - 1) / NT;
     constexpr int NQ = (kRowMax * kRowInts + NT - 1) / NT;  // record ints per thread
     uint32_t* roff = reinterpret_cast<uint32_t*>(lds_raw);
     int* rbuf = reinterpret_cast<int*>(lds_raw + (size_t)kChunkMax * DMB * 4);
@@ -946,9 +949,13 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
         // window or for a channel the last group lacks) summed into presum
         // samples 4g..4g+3.  The row record holds each channel's byte offset
         // and phase.  A thread takes items tid, tid + NT, ...: (r, g) advance
-        // by (NT / ng, NT mod ng) with one carry, and a thread past the last
-        // row loads row 0's elements without writing them.
-        const int rq = NT / ng, rr = NT - rq * ng;
+        // by (NT / ng, NT mod ng) with one carry.
+        int rq = NT / ng, rr = NT - rq * ng, ngv = ng;
+        // Wide f32/f16 tiles keep these in vector registers for the loop: its
+        // carry step would otherwise copy them from scalar registers on every
+        // item (one scalar operand per vector instruction).  The other tiles
+        // have no three registers to spare.
+        if constexpr (WIDE && ESZ != 1) asm volatile("" : "+v"(rq), "+v"(rr), "+v"(ngv));
         int r = (int)((float)tid * (1.0f / (float)ng));
         int g = tid - r * ng;
         {
@@ -957,76 +964,81 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
             g -= fix * ng;
         }
         for (int j0 = 0; j0 < nrw * ng; j0 += NT) {
+            // A thread past the last row: wide tiles skip its item; the
+            // others (fewer registers to spare) load row 0's elements and drop
+            // them, which keeps their staging branch-free.
             const bool ok = r < nrw;
-            const int* rec = rcur + (ok ? r : 0) * kRowInts;
-            const int4 rbase = *reinterpret_cast<const int4*>(rec);
-            const int4 rmeta = *reinterpret_cast<const int4*>(rec + 4);
-            const int bases[kGroupCh] = {rbase.x, rbase.y, rbase.z, rbase.w};
-            const uint32_t lo = toff + (uint32_t)(16 * g / EPD);
-            uint32_t vo[kGroupCh];
-            int ph[kGroupCh];
-            float sc[kGroupCh];
-#pragma unroll
-            for (int c = 0; c < kGroupCh; c++) {
-                vo[c] = (uint32_t)bases[c] + lo;
-                ph[c] = (rmeta.x >> (8 * c)) & 3;
-                sc[c] = 1.0f;
-                if constexpr (ESZ == 1) sc[c] = bases[c] >= 0 ? chan_scale[ch.c0 + rmeta.z + c] : 0.0f;
-            }
-            unsigned char* dst = data + rmeta.y + 16 * g;
-            if constexpr (ESZ == 2) {
-                // f16: three raw dwords per channel, shifted by the phase and
-                // summed by group_sum_f16
-                uint32_t raw[kGroupCh][3];
-#pragma unroll
-                for (int c = 0; c < kGroupCh; c++)
-#pragma unroll
-                    for (int i = 0; i < 3; i++)
-                        raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
-                // all twelve loads in flight before the first use: the
-                // scheduler otherwise waits for each channel in turn
-                __builtin_amdgcn_sched_barrier(0);
-                uint32_t al[kGroupCh][2];
-#pragma unroll
-                for (int c = 0; c < kGroupCh; c++)
-#pragma unroll
-                    for (int i = 0; i < 2; i++) al[c][i] = __builtin_amdgcn_alignbyte(raw[c][i + 1], raw[c][i], 2 * ph[c]);
-                if (ok) *reinterpret_cast<float4*>(dst) = group_sum_f16(al);
-            } else {
-                // f32: four dwords per channel; u8: two, shifted by the phase
-                // and scaled (the elements of load_elems<TS, 4>), all loads
-                // in flight before the first use
-                constexpr int NW = ESZ == 4 ? 4 : 2;
-                uint32_t raw[kGroupCh][NW];
-#pragma unroll
-                for (int c = 0; c < kGroupCh; c++)
-#pragma unroll
-                    for (int i = 0; i < NW; i++)
-                        raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
-                __builtin_amdgcn_sched_barrier(0);
-                float x[kGroupCh][4];
+            if (!WIDE || ok) {
+                const int* rec = rcur + (WIDE || ok ? r : 0) * kRowInts;
+                const int4 rbase = *reinterpret_cast<const int4*>(rec);
+                const int4 rmeta = *reinterpret_cast<const int4*>(rec + 4);
+                const int bases[kGroupCh] = {rbase.x, rbase.y, rbase.z, rbase.w};
+                const uint32_t lo = toff + (uint32_t)(16 * g / EPD);
+                uint32_t vo[kGroupCh];
+                int ph[kGroupCh];
+                float sc[kGroupCh];
 #pragma unroll
                 for (int c = 0; c < kGroupCh; c++) {
-                    if constexpr (ESZ == 4) {
-#pragma unroll
-                        for (int e = 0; e < 4; e++) x[c][e] = __uint_as_float(raw[c][e]);
-                    } else {
-                        const uint32_t wd = __builtin_amdgcn_alignbyte(raw[c][1], raw[c][0], ph[c]);
-#pragma unroll
-                        for (int e = 0; e < 4; e++) x[c][e] = sc[c] * (float)((wd >> (8 * e)) & 0xffu);
-                    }
+                    vo[c] = (uint32_t)bases[c] + lo;
+                    ph[c] = (rmeta.x >> (8 * c)) & 3;
+                    sc[c] = 1.0f;
+                    if constexpr (ESZ == 1) sc[c] = bases[c] >= 0 ? chan_scale[ch.c0 + rmeta.z + c] : 0.0f;
                 }
-                float4 w;
-                w.x = (x[0][0] + x[1][0]) + (x[2][0] + x[3][0]);
-                w.y = (x[0][1] + x[1][1]) + (x[2][1] + x[3][1]);
-                w.z = (x[0][2] + x[1][2]) + (x[2][2] + x[3][2]);
-                w.w = (x[0][3] + x[1][3]) + (x[2][3] + x[3][3]);
-                if (ok) *reinterpret_cast<float4*>(dst) = w;
+                unsigned char* dst = data + rmeta.y + 16 * g;
+                if constexpr (ESZ == 2) {
+                    // f16: three raw dwords per channel, shifted by the phase and
+                    // summed by group_sum_f16
+                    uint32_t raw[kGroupCh][3];
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++)
+#pragma unroll
+                        for (int i = 0; i < 3; i++)
+                            raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
+                    // all twelve loads in flight before the first use: the
+                    // scheduler otherwise waits for each channel in turn
+                    __builtin_amdgcn_sched_barrier(0);
+                    uint32_t al[kGroupCh][2];
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++)
+#pragma unroll
+                        for (int i = 0; i < 2; i++) al[c][i] = __builtin_amdgcn_alignbyte(raw[c][i + 1], raw[c][i], 2 * ph[c]);
+                    if (WIDE || ok) *reinterpret_cast<float4*>(dst) = group_sum_f16(al);
+                } else {
+                    // f32: four dwords per channel; u8: two, shifted by the phase
+                    // and scaled (the elements of load_elems<TS, 4>), all loads
+                    // in flight before the first use
+                    constexpr int NW = ESZ == 4 ? 4 : 2;
+                    uint32_t raw[kGroupCh][NW];
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++)
+#pragma unroll
+                        for (int i = 0; i < NW; i++)
+                            raw[c][i] = (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(rsrc, vo[c] + 4 * i, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    float x[kGroupCh][4];
+#pragma unroll
+                    for (int c = 0; c < kGroupCh; c++) {
+                        if constexpr (ESZ == 4) {
+#pragma unroll
+                            for (int e = 0; e < 4; e++) x[c][e] = __uint_as_float(raw[c][e]);
+                        } else {
+                            const uint32_t wd = __builtin_amdgcn_alignbyte(raw[c][1], raw[c][0], ph[c]);
+#pragma unroll
+                            for (int e = 0; e < 4; e++) x[c][e] = sc[c] * (float)((wd >> (8 * e)) & 0xffu);
+                        }
+                    }
+                    float4 w;
+                    w.x = (x[0][0] + x[1][0]) + (x[2][0] + x[3][0]);
+                    w.y = (x[0][1] + x[1][1]) + (x[2][1] + x[3][1]);
+                    w.z = (x[0][2] + x[1][2]) + (x[2][2] + x[3][2]);
+                    w.w = (x[0][3] + x[1][3]) + (x[2][3] + x[3][3]);
+                    if (WIDE || ok) *reinterpret_cast<float4*>(dst) = w;
+                }
             }
             r += rq;
             g += rr;
-            if (g >= ng) {
-                g -= ng;
+            if (g >= ngv) {
+                g -= ngv;
                 r++;
             }
         }
@@ -1034,7 +1046,7 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int i = tid + NT * r;
-            if (i < ngr * DMB / 4)
+            if (i < ngr * DMB / (4 * OPW))
                 reinterpret_cast<u32x4*>(roff)[i] = reinterpret_cast<const u32x4*>(roffs + ch.roff)[i];
         }
         if (more) {
@@ -1051,11 +1063,15 @@ dedisp_group_tile(const TS* __restrict__ win, int64_t ld, int nch, int nrows, in
             // sample / 4) with 16-byte LDS reads, which run at nearly twice
             // the rate of the two-sample reads below; wide_add picks the add
             // sequence of the start's phase.  Lane 63's sums are not stored.
-            const uint32_t* ro = roff + wave * DW;
+            const uint32_t* ro = roff + wave * DW / 2;
             for (int gi = 0; gi < ngr; gi++) {
                 uint32_t o[DW];
 #pragma unroll
-                for (int d = 0; d < DW; d++) o[d] = __builtin_amdgcn_readfirstlane(ro[gi * DMB + d]);
+                for (int d = 0; d < DW; d += 2) {
+                    const uint32_t w = __builtin_amdgcn_readfirstlane(ro[gi * DMB / 2 + d / 2]);
+                    o[d] = w & 0xffffu;
+                    o[d + 1] = w >> 16;
+                }
                 // the next DM's quads are read before this one's are added
                 float4 v[2][NB];
                 auto quads = [&](int d) {
@@ -1534,13 +1550,20 @@ vfrb_status build_tile_set(const vfrb_search_plan* p, const TileCfg& tc, int i, 
                         }
                         recs.insert(recs.end(), rec, rec + kRowInts);
                     }
+                    // wide tiles: two DMs' offsets per word (low half the even DM),
+                    // each below the 64 KB the data budget stays under
                     for (int d = 0; d < DMB; d++) {
                         uint32_t o = 0;
                         if (d0 + d < d1) {
                             const int a = p->off[(size_t)(d0 + d) * nch + c0] - amin;
                             o = (uint32_t)(((rowstart + rowof[g][d]) * stride + a) * 4);
                         }
-                        roffs.push_back(o);
+                        if (tc.mode == 3) {
+                            if (d % 2 == 0) roffs.push_back(o);
+                            else roffs.back() |= o << 16;
+                        } else {
+                            roffs.push_back(o);
+                        }
                     }
                     rowstart += (int)tup[g].size();
                 }
