@@ -271,6 +271,91 @@ __device__ __forceinline__ void apply_twiddles(cf* v, const float2* __restrict__
     for (int r = 1; r < R; r++) v[r] = cmul(v[r], w[r]);
 }
 
+// Twiddled DFT of a power-of-two length R in one pass of FMA butterflies:
+// y_k = sum_r w^r x_r exp(-2 pi i r k / R) as a radix-2 decimation in time
+// (inputs in bit-reversed order, compile-time indices).  Its stage of size S
+// combines two half-size results of twiddle base w^(R/S) with
+// z_k = w^(R/S) exp(-2 pi i k / S), k < S/2:
+//     y_k = e_k + z_k o_k,   y_(k+S/2) = 2 e_k - y_k,
+// 4 + 2 FMAs per butterfly, against a complex product (4) and two complex
+// additions (4) when the twiddles are applied before a plain DFT.  For
+// k >= S/4, z_k = z_(k-S/4) (-i), which only swaps operands and signs, so
+// a stage needs S/4 stored values (one for S = 2): R/2 per set.
+template <int R>
+struct TwFused {
+    cf t[R / 2];  // stage S: entries [S/4, S/2) for S >= 4, entry 0 for S = 2
+};
+// From the table values w^(2^b): stage S's z_j = w^(R/S) * exp(-2 pi i j/S).
+template <int R>
+__device__ __forceinline__ TwFused<R> fuse(const TwSet<R>& s) {
+    TwFused<R> f;
+    f.t[0] = cf{s.w[__builtin_ctz(R) - 1].x, s.w[__builtin_ctz(R) - 1].y};  // S = 2: w^(R/2)
+#pragma unroll
+    for (int S = 4; S <= R; S *= 2) {
+        const int b = __builtin_ctz(R / S);
+        const cf base = cf{s.w[b].x, s.w[b].y};
+#pragma unroll
+        for (int j = 0; j < S / 4; j++) f.t[S / 4 + j] = j == 0 ? base : cmul(base, wconst<32>(j * (32 / S)));
+    }
+    return f;
+}
+template <int R>
+__device__ __forceinline__ void refresh(TwFused<R>& f) {
+#pragma unroll
+    for (int i = 0; i < R / 2; i++) asm volatile("" : "+v"(f.t[i].x), "+v"(f.t[i].y));
+}
+__host__ __device__ constexpr int bitrev(int x, int bits) {
+    int y = 0;
+    for (int i = 0; i < bits; i++) y |= ((x >> i) & 1) << (bits - 1 - i);
+    return y;
+}
+template <int R>
+__device__ __forceinline__ void tdft(cf* v, const TwFused<R>& f) {
+    constexpr int LB = __builtin_ctz(R);
+    cf a[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) a[r] = v[bitrev(r, LB)];
+#pragma unroll
+    for (int S = 2; S <= R; S *= 2) {
+#pragma unroll
+        for (int blk = 0; blk < R; blk += S) {
+#pragma unroll
+            for (int k = 0; k < S / 2; k++) {
+                const int q = S == 2 ? 0 : S / 4 + (k % (S / 4));
+                const bool rot = S >= 4 && k >= S / 4;  // z_k = z_(k-S/4) (-i)
+                const cf z = f.t[q];
+                const cf e = a[blk + k], o = a[blk + k + S / 2];
+                cf y;
+                if (!rot) {  // e + z o
+                    y.x = fmaf(z.x, o.x, fmaf(-z.y, o.y, e.x));
+                    y.y = fmaf(z.x, o.y, fmaf(z.y, o.x, e.y));
+                } else {     // e - i z o
+                    y.x = fmaf(z.x, o.y, fmaf(z.y, o.x, e.x));
+                    y.y = fmaf(-z.x, o.x, fmaf(z.y, o.y, e.y));
+                }
+                a[blk + k] = y;
+                a[blk + k + S / 2] = cf{fmaf(2.0f, e.x, -y.x), fmaf(2.0f, e.y, -y.y)};
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = a[r];
+}
+// A pass's twiddles as the fused set: formed once (H) or every frame.
+template <bool H, int R>
+__device__ __forceinline__ auto pick_fused(const TwSet<R>& s) {
+    if constexpr (H) return fuse(s);
+    else return s;
+}
+template <int R>
+__device__ __forceinline__ void twiddled_dft(cf* v, const TwFused<R>& f) {
+    tdft<R>(v, f);
+}
+template <int R>
+__device__ __forceinline__ void twiddled_dft(cf* v, const TwSet<R>& s) {
+    tdft<R>(v, fuse(s));
+}
+
 // ---------------------------------------------------------------------------
 // Power-of-two kernel.  32 complex elements per thread, TPF = N/32 threads per
 // FFT, FPB = BLOCK/TPF FFTs of the same output row in flight per workgroup.
@@ -348,8 +433,7 @@ __device__ __forceinline__ void p2_pass(float* __restrict__ re, float* __restric
     for (int q = 0; q < Q; q++) {
         const int j = t + TPF * q;
         const int k = j & (NS - 1);
-        apply_twiddle_set<R>(v[q], ws);  // k == t mod NS for every q
-        dft<R>(v[q]);
+        twiddled_dft<R>(v[q], ws);  // k == t mod NS for every q
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -575,11 +659,11 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
     TwSet<(C::MID1 > 1 ? C::MID1 : 2)> tw_m10{};
     if constexpr (C::MID1 > 1)
         tw_m10 = load_twiddles<C::MID1>(tw, (t & (NS2 - 1)) << (C::LOG2 - __builtin_ctz(NS2 * C::MID1)));
-    auto tw_m0 = pick_twiddles<C::HOIST>(
+    auto tw_m0 = pick_fused<C::HOIST>(
         load_twiddles<C::MID0>(tw, (t & 7) << (C::LOG2 - __builtin_ctz(8 * C::MID0))));
-    auto tw_m1 = pick_twiddles<C::HOIST>(tw_m10);
-    auto tw_a = pick_twiddles<C::HOIST>(load_twiddles<16>(tw, ja));
-    auto tw_b = pick_twiddles<C::HOIST>(load_twiddles<16>(tw, jb));
+    auto tw_m1 = pick_fused<C::HOIST>(tw_m10);
+    auto tw_a = pick_fused<C::HOIST>(load_twiddles<16>(tw, ja));
+    auto tw_b = pick_fused<C::HOIST>(load_twiddles<16>(tw, jb));
     fetch(0);
     for (int it = 0; it < nave; it += C::FPB) {
         refresh(tw_m0);
@@ -639,8 +723,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                                          : C::PadC::idx(fbase + ja + r * NS);
                 a[r] = cf{re[ia], im[ia]};
             }
-            apply_twiddle_set<16>(a, tw_a);
-            dft<16>(a);
+            twiddled_dft<16>(a, tw_a);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 if constexpr (C::SPLIT) sa[r] = fmaf(a[r].x, a[r].x, fmaf(a[r].y, a[r].y, sa[r]));
@@ -655,8 +738,7 @@ fb_pow2_kernel(const uint8_t* __restrict__ pol0, const uint8_t* __restrict__ pol
                                          : C::PadC::idx(fbase + jb + r * NS);
                 b[r] = cf{re[ib], im[ib]};
             }
-            apply_twiddle_set<16>(b, tw_b);
-            dft<16>(b);
+            twiddled_dft<16>(b, tw_b);
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 if constexpr (C::SPLIT) sb[r] = fmaf(b[r].x, b[r].x, fmaf(b[r].y, b[r].y, sb[r]));
