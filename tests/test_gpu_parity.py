@@ -441,6 +441,55 @@ def test_reduced_precision_storage(gpu, orc, storage, tol, rms_tol):
     assert rms < rms_tol, rms
 
 
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_storage_exact_on_representable_window(gpu, storage):
+    """Each storage mode is exact where its format holds the window exactly.
+    f32/f16: integers in [-1000, 1000] (exact in f16, power-of-two scale 1).
+    u8: channel c holds lo_c + 2^-(c mod 4) * k, k in 0..255 with both ends
+    present, lo_c an integer: the range gives scale 2^-(c mod 4) and offset
+    lo_c exactly, so every byte is k.  All partial sums are multiples of 1/8
+    below 2^21, exact in f32, and the channel mean divides by 1024: every
+    output equals the exact dedispersed mean bit for bit, in every tile
+    family (what the reduced-precision tolerances above do not pin)."""
+    vfrb = gpu
+    nch, tchan, fch1, chanbw, dms = _c1_shape()
+    rows = 976
+    rng = np.random.default_rng(12)
+    if storage == "u8":
+        k = rng.integers(0, 256, (rows, nch))
+        k[3] = 0
+        k[7] = 255
+        j = np.arange(nch) % 4
+        lo = rng.integers(-300, 301, nch)
+        units = lo[None, :] * 8 + k * (8 >> j)[None, :]  # window in units of 1/8
+    else:
+        units = rng.integers(-1000, 1001, (rows, nch)) * 8
+    fb = (units / 8.0).astype(np.float32)
+    assert np.array_equal(fb.astype(np.float64) * 8, units)
+    names = vfrb.tile_names()
+    plan = vfrb.SearchPlan(nch, tchan, fch1, chanbw, dms, rows, vfrb.STORAGE_NAMES[storage])
+    nt = rows - plan.max_delay
+    off = plan.offsets().astype(np.int64)
+    cols = np.arange(nch)
+    t = np.arange(nt)
+    exp = np.empty((len(dms), nt), np.float32)
+    for d in range(len(dms)):
+        s = units[t[:, None] + off[d][None, :], cols[None, :]].sum(axis=1)
+        exp[d] = (s / 8.0 / nch).astype(np.float32)
+        assert np.array_equal(exp[d].astype(np.float64) * 8 * nch, s)  # exact in f32
+    d_fb = vfrb.DeviceArray.from_host(fb)
+    plan.load(d_fb, rows, bandpass=False)
+    out = vfrb.DeviceArray((len(dms), nt), np.float32)
+    for tile in ("T512_dm64_w16", "P512_dm64_w16", "G256_dm64_w16", "W504_dm64_w16", "W252_dm64_w16"):
+        print(f"[case] exact window {storage} tile {tile}", flush=True)
+        plan.set_tile(names.index(tile))
+        out.zero()
+        plan.dedisperse(rows, out, nt)
+        got = out.download()
+        bad = np.argwhere(got != exp)
+        assert bad.size == 0, (storage, tile, bad[:5], got[tuple(bad[0])], exp[tuple(bad[0])])
+
+
 def test_dedisperse_stats_variant(gpu, orc):
     vfrb = gpu
     nch, tchan, fch1, chanbw = 3840, 4 * 12500 / 128e6, 64.0 / 6250, None
