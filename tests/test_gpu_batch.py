@@ -70,11 +70,12 @@ def test_batch_equals_single_plans(gpu, orc, storage, tile):
             assert np.max(np.abs(g - ref)) < 2e-3
 
 
-def test_batch_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_batch_graph_replays_match_eager(gpu, storage):
     vfrb = gpu
-    print("[case] batch graph replay", flush=True)
-    jobs, fbs = _make_jobs(vfrb, "f16", seed=12)
-    batch = vfrb.Batch(vfrb.STORE_F16)
+    print(f"[case] batch graph replay {storage}", flush=True)
+    jobs, fbs = _make_jobs(vfrb, storage, seed=12)
+    batch = vfrb.Batch(vfrb.STORAGE_NAMES[storage])
     s = vfrb.Stream()
     batch.set(jobs, stream=s)
     batch.dedisperse(stream=s)

@@ -11,7 +11,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
 def test_captured_chain_replays_equal_eager(gpu, storage):
     vfrb = gpu
     n, nave = 2048, 64
