@@ -106,28 +106,29 @@ def test_server_windows_match_oracle_replay(gpu, orc):
     assert searched >= 4 and pulsed == 1
 
 
-def test_server_result_is_the_same_alone_and_in_the_mix(gpu):
+@pytest.mark.parametrize("storage", ["f32", "f16", "u8"])
+def test_server_result_is_the_same_alone_and_in_the_mix(gpu, storage):
     vfrb = gpu
     p = _shape(vfrb)
     a = _stream_bytes(vfrb, p, seed=5, pulse_t0=0.12)
     others = [_stream_bytes(vfrb, p, seed=s, pulse_t0=0.05 * s) for s in (6, 7, 8)]
-    print("[case] server f16: stream alone (batched)", flush=True)
-    s1 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    print(f"[case] server {storage}: stream alone (batched)", flush=True)
+    s1 = _server(vfrb, p, vfrb.STORAGE_NAMES[storage], batched=True)
     sid = s1.open_stream(DMS)
     alone = _run(s1, [(sid, *a, 0)])
     s1.close()
-    print("[case] server f16: stream alone (plain)", flush=True)
-    s2 = _server(vfrb, p, vfrb.STORE_F16, batched=False)
+    print(f"[case] server {storage}: stream alone (plain)", flush=True)
+    s2 = _server(vfrb, p, vfrb.STORAGE_NAMES[storage], batched=False)
     sid = s2.open_stream(DMS)
     plain = _run(s2, [(sid, *a, 0)])
     s2.close()
-    print("[case] server f16: stream alone (batched, device-resident payload)", flush=True)
-    s4 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    print(f"[case] server {storage}: stream alone (batched, device-resident payload)", flush=True)
+    s4 = _server(vfrb, p, vfrb.STORAGE_NAMES[storage], batched=True)
     sid = s4.open_stream(DMS)
     dev = _run(s4, [(sid, *a, 0)], device=True)
     s4.close()
-    print("[case] server f16: stream in a mix of four", flush=True)
-    s3 = _server(vfrb, p, vfrb.STORE_F16, batched=True)
+    print(f"[case] server {storage}: stream in a mix of four", flush=True)
+    s3 = _server(vfrb, p, vfrb.STORAGE_NAMES[storage], batched=True)
     ids = [s3.open_stream(DMS) for _ in range(4)]
     # stream A is the second one opened; the others get their own DM lists
     streams = [(ids[0], *others[0], 100), (ids[1], *a, 0), (ids[2], *others[1], 200),
